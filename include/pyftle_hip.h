@@ -1,0 +1,164 @@
+/*
+ * pyftle_hip.h -- C ABI of the MI355X FTLE engine (libpyftle_hip.so).
+ *
+ * Every entry point takes plain device pointers and sizes, is enqueued on the
+ * HIP stream passed as `stream` (NULL = the null stream), and returns
+ * PYFTLE_OK (0) or a negative error code; pyftle_last_error() then holds the
+ * message (thread-local).  Argument errors are detected on the host before any
+ * launch.  Nothing is allocated on the device by these calls unless noted.
+ *
+ * Reference interfaces replaced (pyFTLE 1.0.3, paths under src/pyftle/):
+ *   grid_interp/GridInterp.cpp:93-142  interp2d_vec_inplace  -> pyftle_interp2d_vec
+ *   grid_interp/GridInterp.cpp:15-91   interp3d_vec_inplace  -> pyftle_interp3d_vec
+ *   interpolate.py:342-472  HighPerfInterpolator layout/update -> pyftle_pack_snapshot
+ *   interpolate.py:475-543  HighPerfInterpolator.interpolate   -> pyftle_field_sample
+ *   integrate.py:161-342    Euler/AB2/RK4 .integrate(h, p)     -> pyftle_advect
+ *   ftle_solver.py:68-161   FTLESolver.run window + _compute_ftle -> pyftle_advect_ftle
+ *   cauchy_green.py:15-210  compute_flow_map_jacobian_2x2/3x3  -> pyftle_flow_map_jacobian
+ *   ftle.py:9-96            compute_cauchy_green_2x2/3x3       -> pyftle_cauchy_green
+ *   ftle.py:99-163          max_eigenvalue_2x2/3x3             -> pyftle_max_eigenvalue
+ *   ftle.py:166-233         compute_ftle_2x2/3x3               -> pyftle_ftle_from_jacobian
+ *   cauchy_green.py + ftle.py (fused)                          -> pyftle_ftle_from_groups
+ */
+#ifndef PYFTLE_HIP_H
+#define PYFTLE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define PYFTLE_OK 0
+#define PYFTLE_EINVAL (-1) /* bad argument (shape, dtype, pointer); maps to ValueError/RuntimeError */
+#define PYFTLE_EHIP (-2)   /* HIP runtime error at launch */
+
+/* ---- element types ---- */
+#define PYFTLE_F32 0
+#define PYFTLE_F64 1
+
+/* ---- snapshot record layouts in the device ring ---- */
+#define PYFTLE_LAYOUT_SINGLE 0 /* record = one time level: 2D (u,v); 3D (u,v,w,0) */
+#define PYFTLE_LAYOUT_PAIR 1   /* record = levels k,k+1: 2D (u0,v0,u1,v1); 3D (u0,v0,w0,0,u1,v1,w1,0) */
+
+/* ---- integrators (integrate.py:350-387 names) ---- */
+#define PYFTLE_EULER 0
+#define PYFTLE_AB2 1
+#define PYFTLE_RK4 2
+
+/* ---- time treatment of the velocity field ---- */
+#define PYFTLE_TIME_STEADY 0 /* every stage reads snapshot `first_snapshot` */
+#define PYFTLE_TIME_FROZEN 1 /* step i reads snapshot first_snapshot+i for all stages (ftle_solver.py:99-102) */
+#define PYFTLE_TIME_LINEAR 2 /* stage time t blends snapshots k,k+1 linearly (needs PYFTLE_LAYOUT_PAIR) */
+
+/* ---- work-to-lane order for grouped kernels ---- */
+#define PYFTLE_ORDER_SEED_MAJOR 0     /* the nb particles of a seed sit in adjacent lanes */
+#define PYFTLE_ORDER_NEIGHBOR_MAJOR 1 /* a wave holds one neighbour direction of consecutive seeds */
+
+/*
+ * A gridded velocity field resident in HBM as a ring of snapshot slots.
+ * Node (ix,iy[,iz]) of slot s is record  ix + n[0]*(iy + n[1]*iz)  at
+ * data + s*slot_stride; snapshot (or pair) k lives in slot k % ring_slots.
+ * Physical -> index mapping is (p - origin)/spacing (grid_interp.py:78-79).
+ */
+typedef struct pyftle_field {
+  int32_t dim;          /* 2 or 3 */
+  int32_t n[3];         /* grid points along x, y, z (n[2] = 1 in 2D) */
+  double origin[3];     /* grid minimum per axis */
+  double spacing[3];    /* (max - min)/(n - 1) per axis (grid_interp.py:34-35) */
+  int32_t periodic[3];  /* 1: index wraps modulo n (extension); 0: zero outside (GridInterp.cpp:58,126) */
+  int32_t dtype;        /* PYFTLE_F32 or PYFTLE_F64 record components; also the compute precision */
+  int32_t layout;       /* PYFTLE_LAYOUT_SINGLE or PYFTLE_LAYOUT_PAIR */
+  int32_t ring_slots;   /* number of resident slots (>= 1) */
+  int32_t n_snapshots;  /* snapshots in the time series (time clamping in LINEAR mode) */
+  int64_t slot_stride;  /* bytes between slots (>= record bytes * nodes) */
+  double t0;            /* time of snapshot 0 (LINEAR mode) */
+  double dt_snap;       /* time between snapshots (LINEAR mode, non-zero) */
+  void *data;           /* device pointer to slot 0 */
+} pyftle_field;
+
+typedef struct pyftle_advect_opts {
+  int32_t method;         /* PYFTLE_EULER / PYFTLE_AB2 / PYFTLE_RK4 */
+  int32_t time_mode;      /* PYFTLE_TIME_* */
+  int32_t n_steps;        /* steps taken in this launch (>= 0) */
+  int32_t first_snapshot; /* STEADY: snapshot read; FROZEN: snapshot of step 0 */
+  double t_start;         /* LINEAR: time of step 0 */
+  double h;               /* signed step size */
+  int32_t ab2_have_prev;  /* AB2: 1 if `vprev` already holds the previous velocity */
+  int32_t lane_order;     /* PYFTLE_ORDER_* (grouped kernels) */
+  int32_t groups_per_block; /* seeds per work-group (grouped kernels), 0 = default */
+  int32_t xcd_remap;      /* 1: map consecutive work to one XCD's L2 (8 XCDs) */
+} pyftle_advect_opts;
+
+/* ---- library ---- */
+int pyftle_version(void);             /* 10000*major + 100*minor + patch */
+const char *pyftle_last_error(void);  /* message of the last failing call on this thread */
+int pyftle_device_sync(void);         /* hipDeviceSynchronize, for tests */
+
+/* ---- gather-only interpolation: GridInterp.cpp:93-142 / 15-91 ----
+ * v (nx,ny) or (X,Y,Z) row-major FP64, points (n,dim) in index space, out (n,).
+ * Same arithmetic and evaluation order as the reference (bit-exact in FP64);
+ * zero outside [0,n-2] cells, including a point exactly on the upper edge. */
+int pyftle_interp2d_vec(const double *v, int32_t nx, int32_t ny, const double *points,
+                        int64_t n, double *out, void *stream);
+int pyftle_interp3d_vec(const double *v, int32_t X, int32_t Y, int32_t Z, const double *points,
+                        int64_t n, double *out, void *stream);
+
+/* ---- snapshot ingest: interpolate.py:342-472 (layout of one snapshot) ----
+ * Copies snapshot `snapshot` from a component-planar source into the ring.
+ * Source element (c, ix, iy, iz) is at src[c*s[0] + ix*s[1] + iy*s[2] + iz*s[3]]
+ * (element strides), so the reference's 2D F-order and 3D axis-swapped layouts
+ * are both expressed by strides.  PAIR layout writes level 0 of pair slot k and
+ * level 1 of pair slot k-1 (and level 1 of slot k when k is the last snapshot). */
+int pyftle_pack_snapshot(const pyftle_field *field, int32_t snapshot, const void *src,
+                         int32_t src_dtype, const int64_t src_strides[4], void *stream);
+
+/* ---- physical-space sampling: HighPerfInterpolator.interpolate (interpolate.py:475-543)
+ * points (n,dim) of `pts_dtype`, out (n,dim) of the same dtype; velocity at time
+ * t (LINEAR) or of snapshot `snapshot` (STEADY/FROZEN). */
+int pyftle_field_sample(const pyftle_field *field, int32_t time_mode, int32_t snapshot, double t,
+                        const void *points, int32_t pts_dtype, int64_t n, void *out, void *stream);
+
+/* ---- advection: Integrator.integrate (integrate.py:161-342), n_steps at once ----
+ * positions (n_particles, dim) of pos_dtype, updated in place.  vprev (same
+ * shape/dtype) is the AB2 state (may be NULL for Euler/RK4).  Compute runs in
+ * the field's dtype; FP32 keeps a compensated (hi, lo) position. */
+int pyftle_advect(const pyftle_field *field, const pyftle_advect_opts *opts, void *positions,
+                  int32_t pos_dtype, int64_t n_particles, void *vprev, void *stream);
+
+/* ---- fused window: FTLESolver.run + _compute_ftle (ftle_solver.py:68-161) ----
+ * positions_in: block-ordered (nb*n_groups, dim) [L,R,T,B(,F,Bk)] of pos_dtype
+ * (particles.py:170,191,212); nb = 2*dim.  Advects every particle for
+ * opts->n_steps, then per seed forms J, C and lambda_max in FP64 and writes
+ * ftle_out[s] = (1/map_period) * log(sqrt(lambda_max)).  positions_out (may be
+ * NULL, may equal positions_in) receives the final positions. */
+int pyftle_advect_ftle(const pyftle_field *field, const pyftle_advect_opts *opts,
+                       const void *positions_in, int32_t pos_dtype, int64_t n_groups,
+                       void *positions_out, double map_period, double *ftle_out, void *stream);
+
+/* ---- FTLE tensors (FP64 in/out) ----
+ * pyftle_flow_map_jacobian: cauchy_green.py:15-153.  positions / initial
+ *   positions block-ordered (nb*n, dim) of pos_dtype -> J (n,dim,dim).
+ * pyftle_cauchy_green: ftle.py:9-96 (2D F^T F, 3D F F^T) -> (n,dim,dim).
+ * pyftle_max_eigenvalue: ftle.py:99-163; 2D closed form for a general 2x2
+ *   (NaN for complex pairs, as the reference), 3D symmetric trigonometric form.
+ * pyftle_ftle_from_jacobian: ftle.py:166-233.
+ * pyftle_ftle_from_groups: Jacobian + FTLE fused. */
+int pyftle_flow_map_jacobian(const void *positions, const void *initial_positions,
+                             int32_t pos_dtype, int32_t dim, int64_t n, double *J_out,
+                             void *stream);
+int pyftle_cauchy_green(const double *J, int32_t dim, int64_t n, double *C_out, void *stream);
+int pyftle_max_eigenvalue(const double *C, int32_t dim, int64_t n, double *lam_out,
+                          void *stream);
+int pyftle_ftle_from_jacobian(const double *J, int32_t dim, int64_t n, double map_period,
+                              double *ftle_out, void *stream);
+int pyftle_ftle_from_groups(const void *positions, const void *initial_positions,
+                            int32_t pos_dtype, int32_t dim, int64_t n, double map_period,
+                            double *ftle_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYFTLE_HIP_H */

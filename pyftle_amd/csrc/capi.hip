@@ -1,0 +1,89 @@
+// Library-level C ABI: version, thread-local error text, field validation.
+#include <math.h>
+
+#include <cstdio>
+#include <string>
+
+#include "common.hpp"
+
+namespace pyftle {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(const std::string &msg, int code) {
+  set_error(msg);
+  return code;
+}
+
+int check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    return fail(std::string(what) + ": " + hipGetErrorString(e), PYFTLE_EHIP);
+  }
+  return PYFTLE_OK;
+}
+
+int validate_field(const pyftle_field *f) {
+  if (!f) return fail("field is NULL");
+  if (f->dim != 2 && f->dim != 3) return fail("field.dim must be 2 or 3");
+  for (int a = 0; a < f->dim; ++a) {
+    if (f->n[a] < 2) return fail("field.n must be >= 2 on every axis");
+    if (!(f->spacing[a] != 0.0) || !isfinite(f->spacing[a]))
+      return fail("field.spacing must be finite and non-zero");
+    if (!isfinite(f->origin[a])) return fail("field.origin must be finite");
+  }
+  if (f->dtype != PYFTLE_F32 && f->dtype != PYFTLE_F64) return fail("field.dtype must be F32 or F64");
+  if (f->layout != PYFTLE_LAYOUT_SINGLE && f->layout != PYFTLE_LAYOUT_PAIR)
+    return fail("field.layout must be SINGLE or PAIR");
+  if (f->ring_slots < 1) return fail("field.ring_slots must be >= 1");
+  if (f->n_snapshots < 1) return fail("field.n_snapshots must be >= 1");
+  if (!f->data) return fail("field.data is NULL");
+  int64_t nodes = (int64_t)f->n[0] * f->n[1] * (f->dim == 3 ? f->n[2] : 1);
+  if (nodes > (int64_t)INT32_MAX / 2) return fail("field grid too large (> 2^30 nodes)");
+  int64_t need = nodes * record_bytes(f->dim, f->layout, f->dtype);
+  if (f->slot_stride < need) return fail("field.slot_stride smaller than one slot of records");
+  if (f->slot_stride % 16 != 0) return fail("field.slot_stride must be a multiple of 16 bytes");
+  if ((reinterpret_cast<uintptr_t>(f->data) % 32) != 0) return fail("field.data must be 32-byte aligned");
+  return PYFTLE_OK;
+}
+
+int make_field_dev(const pyftle_field *f, FieldDev *d) {
+  int rc = validate_field(f);
+  if (rc) return rc;
+  d->data = reinterpret_cast<const char *>(f->data);
+  d->slot_stride = f->slot_stride;
+  d->ring = f->ring_slots;
+  d->nsnap = f->n_snapshots;
+  for (int a = 0; a < 3; ++a) {
+    const bool used = a < f->dim;
+    d->n[a] = used ? f->n[a] : 1;
+    d->periodic[a] = used ? (f->periodic[a] ? 1 : 0) : 0;
+    d->origin[a] = used ? f->origin[a] : 0.0;
+    d->spacing[a] = used ? f->spacing[a] : 1.0;
+    d->origin_f[a] = (float)d->origin[a];
+    d->inv_spacing_f[a] = (float)(1.0 / d->spacing[a]);
+    d->n_f[a] = (float)d->n[a];
+    d->inv_n_f[a] = (float)(1.0 / d->n[a]);
+  }
+  d->t0 = f->t0;
+  d->dts = f->dt_snap;
+  return PYFTLE_OK;
+}
+
+}  // namespace pyftle
+
+extern "C" {
+
+int pyftle_version(void) { return 10000 * 0 + 100 * 1 + 0; }
+
+const char *pyftle_last_error(void) { return pyftle::g_last_error.c_str(); }
+
+int pyftle_device_sync(void) {
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return pyftle::fail(std::string("hipDeviceSynchronize: ") + hipGetErrorString(e), PYFTLE_EHIP);
+  return PYFTLE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,234 @@
+"""Device-resident velocity fields: a ring of snapshot slots in HBM.
+
+A ``DeviceField`` owns one HBM buffer holding ``ring_slots`` snapshot slots of
+interleaved records (2D ``(u, v)``, 3D ``(u, v, w, 0)``; ``PAIR`` layout packs
+snapshots ``k`` and ``k+1`` into one record so a time-linear lookup fetches
+both levels in one load).  Node ``(ix, iy, iz)`` is record
+``ix + nx*(iy + ny*iz)``.  Snapshots are copied in by the HIP pack kernel
+from the component-planar arrays pyFTLE hands to ``Interpolator.update``
+(interpolate.py:70-94, 446-472), whose flat layouts are expressed as strides:
+
+* 2D: ``velocities.reshape((2, nx, ny), order="F")`` (interpolate.py:363),
+  element ``(c, ix, iy)`` at ``c*N + ix + nx*iy``;
+* 3D: ``transpose(velocities.reshape((3, nx, ny, nz)), (0, 2, 1, 3))``
+  (interpolate.py:383-385); the C++ kernel then sees an ``(ny, nx, nz)``
+  array, element ``(c, i, j, k)`` at ``c*N + j*ny*nz + i*nz + k``.
+
+Grid geometry follows interpolate.py:359-401 and grid_interp.py:30-37:
+axes ``linspace(min(coord), max(coord), n)`` and spacing
+``(max - min)/(n - 1)``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+ALIGN = 256
+
+
+def cuda_device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("pyftle_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_handle(stream: torch.cuda.Stream | None = None) -> ctypes.c_void_p:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def dtype_code(dtype: torch.dtype) -> int:
+    if dtype == torch.float32:
+        return N.F32
+    if dtype == torch.float64:
+        return N.F64
+    raise ValueError(f"unsupported dtype {dtype}; use float32 or float64")
+
+
+def precision_dtype(precision) -> torch.dtype:
+    if precision in (torch.float32, "float32", "f32", "fp32", np.float32):
+        return torch.float32
+    if precision in (torch.float64, "float64", "f64", "fp64", np.float64):
+        return torch.float64
+    raise ValueError(f"unknown precision {precision!r}")
+
+
+def ptr(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def to_device(x, dtype: torch.dtype | None = None) -> torch.Tensor:
+    dev = cuda_device()
+    if isinstance(x, torch.Tensor):
+        t = x.to(device=dev, dtype=dtype if dtype is not None else x.dtype)
+    else:
+        arr = np.asarray(x)
+        t = torch.from_numpy(np.ascontiguousarray(arr)).to(device=dev)
+        if dtype is not None:
+            t = t.to(dtype)
+    return t.contiguous()
+
+
+@dataclass
+class GridGeometry:
+    """Array extents, origin and spacing per axis (device index order x, y, z)."""
+
+    dim: int
+    n: tuple
+    origin: tuple
+    spacing: tuple
+    periodic: tuple
+    src_strides: tuple  # element strides (c, x, y, z) of the reference flat layout
+    grid_shape: tuple   # the user's grid_shape
+
+    @staticmethod
+    def from_reference(grid_shape, points, periodic=None) -> "GridGeometry":
+        """Geometry of ``HighPerfInterpolator._initialize_interpolator``.
+
+        ``points`` is the ``(dim, Ngrid)`` coordinate array of one snapshot
+        (interpolate.py:359, 365-401).
+        """
+        grid_shape = tuple(int(s) for s in grid_shape)
+        dim = len(grid_shape)
+        if dim not in (2, 3):
+            raise ValueError(f"Unsupported grid dimensionality: {dim}")
+        pts = points.detach().cpu().numpy() if isinstance(points, torch.Tensor) else np.asarray(points)
+        pts = pts.astype(np.float64, copy=False)
+        ngrid = int(np.prod(grid_shape))
+        if pts.shape != (dim, ngrid):
+            raise ValueError(f"points must have shape {(dim, ngrid)}, got {pts.shape}")
+        coords = pts.reshape((dim, *grid_shape))
+        origin, spacing = [], []
+        for a in range(dim):
+            axis = np.linspace(np.min(coords[a]), np.max(coords[a]), grid_shape[a])
+            origin.append(float(axis[0]))
+            spacing.append(float((axis[-1] - axis[0]) / (axis.shape[0] - 1)))
+        if dim == 2:
+            nx, ny = grid_shape
+            n = (nx, ny)
+            strides = (ngrid, 1, nx, 0)
+        else:
+            nx, ny, nz = grid_shape
+            n = (ny, nx, nz)  # extents of the transposed array the C++ kernel indexes
+            strides = (ngrid, nz, ny * nz, 1)
+        per = tuple(bool(p) for p in periodic) if periodic is not None else (False,) * dim
+        return GridGeometry(dim, n, tuple(origin), tuple(spacing), per, strides, grid_shape)
+
+    @staticmethod
+    def uniform(n, origin, spacing, periodic=None) -> "GridGeometry":
+        """Geometry of a grid given directly in device order (x fastest).
+
+        Snapshots for it are ``(dim, nz, ny, nx)`` component-planar arrays
+        (or ``(dim, ny, nx)`` in 2D), i.e. flat ``ix + nx*(iy + ny*iz)``.
+        """
+        n = tuple(int(v) for v in n)
+        dim = len(n)
+        ngrid = int(np.prod(n))
+        if dim == 2:
+            strides = (ngrid, 1, n[0], 0)
+        else:
+            strides = (ngrid, 1, n[0], n[0] * n[1])
+        per = tuple(bool(p) for p in periodic) if periodic is not None else (False,) * dim
+        return GridGeometry(dim, n, tuple(float(o) for o in origin), tuple(float(s) for s in spacing),
+                            per, strides, n)
+
+
+class DeviceField:
+    """Ring of snapshot slots resident in HBM plus its C descriptor."""
+
+    def __init__(self, geom: GridGeometry, *, dtype=torch.float32, layout=N.LAYOUT_SINGLE,
+                 ring_slots: int = 1, n_snapshots: int = 1, t0: float = 0.0, dt_snap: float = 1.0):
+        self.geom = geom
+        self.dtype = precision_dtype(dtype)
+        self.layout = layout
+        self.ring_slots = int(ring_slots)
+        self.n_snapshots = int(n_snapshots)
+        self.t0 = float(t0)
+        self.dt_snap = float(dt_snap)
+        dim = geom.dim
+        comps = (2 if dim == 2 else 4) * (2 if layout == N.LAYOUT_PAIR else 1)
+        esize = 4 if self.dtype == torch.float32 else 8
+        nodes = int(np.prod(geom.n))
+        self.record_bytes = comps * esize
+        self.slot_stride = -(-nodes * self.record_bytes // ALIGN) * ALIGN
+        total = self.slot_stride * self.ring_slots + ALIGN
+        self._buf = torch.empty(total, dtype=torch.uint8, device=cuda_device())
+        base = self._buf.data_ptr()
+        self._offset = (-base) % ALIGN
+        self.data_ptr = base + self._offset
+        self.resident = {}  # slot -> snapshot index currently stored
+        self._desc = self._make_desc()
+
+    @property
+    def nbytes(self) -> int:
+        return self.slot_stride * self.ring_slots
+
+    def _make_desc(self) -> N.Field:
+        g = self.geom
+        d = N.Field()
+        d.dim = g.dim
+        for a in range(3):
+            d.n[a] = g.n[a] if a < g.dim else 1
+            d.origin[a] = g.origin[a] if a < g.dim else 0.0
+            d.spacing[a] = g.spacing[a] if a < g.dim else 1.0
+            d.periodic[a] = int(g.periodic[a]) if a < g.dim else 0
+        d.dtype = dtype_code(self.dtype)
+        d.layout = self.layout
+        d.ring_slots = self.ring_slots
+        d.n_snapshots = self.n_snapshots
+        d.slot_stride = self.slot_stride
+        d.t0 = self.t0
+        d.dt_snap = self.dt_snap
+        d.data = self.data_ptr
+        return d
+
+    @property
+    def desc(self) -> N.Field:
+        return self._desc
+
+    def set_time_axis(self, n_snapshots: int, t0: float = 0.0, dt_snap: float = 1.0) -> None:
+        self.n_snapshots = int(n_snapshots)
+        self.t0 = float(t0)
+        self.dt_snap = float(dt_snap)
+        self._desc = self._make_desc()
+
+    def slot_of(self, k: int) -> int:
+        return k % self.ring_slots
+
+    def pack(self, k: int, velocities, strides=None, stream=None) -> None:
+        """Copy snapshot ``k`` (component-planar, reference layout) into its slot."""
+        src = velocities if isinstance(velocities, torch.Tensor) else to_device(velocities)
+        if not src.is_cuda:
+            src = src.to(cuda_device())
+        if src.dtype not in (torch.float32, torch.float64):
+            src = src.to(torch.float64)
+        src = src.contiguous()
+        st = strides if strides is not None else self.geom.src_strides
+        n = list(self.geom.n) + [1] * (3 - self.geom.dim)
+        extent = (self.geom.dim - 1) * st[0] + sum((n[a] - 1) * st[a + 1] for a in range(3)) + 1
+        if any(s < 0 for s in st) or src.numel() < extent:
+            raise ValueError(f"snapshot source has {src.numel()} elements; its strides need {extent}")
+        arr = (ctypes.c_int64 * 4)(*[int(s) for s in st])
+        N.call("pyftle_pack_snapshot", ctypes.byref(self._desc), int(k), ptr(src),
+               dtype_code(src.dtype), arr, stream_handle(stream))
+        self.resident[self.slot_of(k)] = k
+        self._keepalive = src  # the copy is asynchronous: keep the source alive
+
+    def sample(self, points, time_mode=N.TIME_STEADY, snapshot=0, t=0.0):
+        """Velocity at physical points (``HighPerfInterpolator.interpolate``)."""
+        is_np = not isinstance(points, torch.Tensor)
+        pts = to_device(points, torch.float64 if is_np else None)
+        if pts.dtype not in (torch.float32, torch.float64):
+            pts = pts.to(torch.float64)
+        out = torch.empty_like(pts)
+        N.call("pyftle_field_sample", ctypes.byref(self._desc), int(time_mode), int(snapshot),
+               float(t), ptr(pts), dtype_code(pts.dtype), pts.shape[0], ptr(out), stream_handle())
+        if is_np:
+            return out.cpu().numpy()
+        return out

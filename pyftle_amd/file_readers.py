@@ -1,0 +1,56 @@
+"""MATLAB ``.mat`` readers (file_readers.py:35-182), same keys and layouts.
+
+* ``read_velocity``: ``velocity_x``, ``velocity_y`` (, ``velocity_z``),
+  each raveled, stacked to ``(dim, N)``;
+* ``read_coordinate``: ``coordinate_x``, ... likewise;
+* ``read_seed_particles_coordinates``: blocks ``left, right, top, bottom``
+  (, ``front, back``) concatenated along axis 0 into ``NeighboringParticles``.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+from scipy.io import loadmat
+
+from .particles import NeighboringParticles
+
+
+def read_velocity(file_path: str) -> np.ndarray:
+    data = loadmat(file_path)
+    if "velocity_x" not in data or "velocity_y" not in data:
+        raise ValueError(
+            "The MATLAB file does not contain the expected keys 'velocity_x' and"
+            "'velocity_y'."
+        )
+    vx = np.asarray(data["velocity_x"]).ravel()
+    vy = np.asarray(data["velocity_y"]).ravel()
+    if "velocity_z" in data:
+        return np.stack((vx, vy, np.asarray(data["velocity_z"]).ravel()))
+    return np.stack((vx, vy))
+
+
+def read_coordinate(file_path: str) -> np.ndarray:
+    data = loadmat(file_path)
+    if "coordinate_x" not in data or "coordinate_y" not in data:
+        raise ValueError(
+            "The MATLAB file does not contain the expected keys 'coordinate_x' and"
+            "'coordinate_y'."
+        )
+    cx = np.asarray(data["coordinate_x"]).ravel()
+    cy = np.asarray(data["coordinate_y"]).ravel()
+    if "coordinate_z" in data:
+        return np.stack((cx, cy, np.asarray(data["coordinate_z"]).ravel()))
+    return np.stack((cx, cy))
+
+
+def read_seed_particles_coordinates(file_path: str) -> NeighboringParticles:
+    data = loadmat(file_path)
+    if "left" not in data or "right" not in data or "top" not in data or "bottom" not in data:
+        raise ValueError(
+            "The MATLAB file must contain at least "
+            "'left', 'right', 'top' and 'bottom' keys. ('front' and 'back' for 3D)"
+        )
+    blocks = [np.asarray(data[k]) for k in ("left", "right", "top", "bottom")]
+    if "front" in data and "back" in data:
+        blocks += [np.asarray(data["front"]), np.asarray(data["back"])]
+    return NeighboringParticles(positions=np.concatenate(blocks, axis=0))

@@ -1,0 +1,217 @@
+"""GPU parity of the fused window kernel and the drop-in API against the oracle.
+
+FP64 tolerance: positions bit-exact with the oracle (same operation order,
+no contraction); FTLE to 1e-12 relative (device log/sqrt, 3D eigen-solver).
+FP32 tolerance: FTLE max abs error <= 1e-3 * (max - min) of the FP64 field.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import numpy_oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _lib():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from pyftle_amd import _native
+
+    _native.load()
+
+
+def _field_2d(vels, pts, grid_shape, precision, pair=False, dt_snap=0.1):
+    from pyftle_amd import _native as N
+    from pyftle_amd.field import DeviceField, GridGeometry
+
+    geom = GridGeometry.from_reference(grid_shape, pts)
+    f = DeviceField(geom, dtype=precision, layout=N.LAYOUT_PAIR if pair else N.LAYOUT_SINGLE,
+                    ring_slots=len(vels), n_snapshots=len(vels), t0=0.0, dt_snap=dt_snap)
+    for k, v in enumerate(vels):
+        f.pack(k, v)
+    return f
+
+
+def _rel_range_err(got, exp):
+    rng = np.nanmax(exp) - np.nanmin(exp)
+    return np.nanmax(np.abs(got - exp)) / rng
+
+
+@pytest.mark.parametrize("method", ["rk4", "euler", "ab2"])
+def test_window_reference_mode_fp64_bit_exact(method):
+    from pyftle_amd import window as W
+
+    vels, pts = H.double_gyre_window(nx=64, ny=32, n_snap=11)
+    x0 = H.seeds_2d(16, 8)
+    fields = H.grid_fields((64, 32), vels, pts)
+    exp_ftle, exp_pos = O.run_window_reference([f.interpolate for f in fields], x0, 0.1, method)
+    f = _field_2d(vels, pts, (64, 32), "float64")
+    pos = torch.as_tensor(x0, device="cuda")
+    out = torch.empty_like(pos)
+    ftle = W.advect_ftle(f, pos, method=method, time_mode="frozen", n_steps=11, h=0.1,
+                         map_period=10 * 0.1, positions_out=out)
+    np.testing.assert_array_equal(out.cpu().numpy(), exp_pos)
+    np.testing.assert_allclose(ftle.cpu().numpy(), exp_ftle, rtol=1e-12, atol=1e-14)
+
+
+def test_window_reference_mode_fp32_tolerance():
+    from pyftle_amd import window as W
+
+    vels, pts = H.double_gyre_window(nx=128, ny=64, n_snap=21)
+    x0 = H.seeds_2d(40, 20)
+    fields = H.grid_fields((128, 64), vels, pts)
+    exp_ftle, _ = O.run_window_reference([f.interpolate for f in fields], x0, 0.1, "rk4")
+    f = _field_2d(vels, pts, (128, 64), "float32")
+    pos = torch.as_tensor(x0, device="cuda")
+    ftle = W.advect_ftle(f, pos, method="rk4", time_mode="frozen", n_steps=21, h=0.1, map_period=2.0)
+    err = _rel_range_err(ftle.cpu().numpy(), exp_ftle)
+    print(f"[fp32 frozen 2D] max|err|/range = {err:.3e}")
+    assert err <= 1e-3
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_window_time_linear_2d(precision):
+    from pyftle_amd import window as W
+
+    vels, pts = H.double_gyre_window(nx=96, ny=48, n_snap=21)
+    x0 = H.seeds_2d(24, 12)
+    fields = H.grid_fields((96, 48), vels, pts)
+    vat = O.time_linear_velocity([f.interpolate for f in fields], 0.0, 0.1)
+    exp_ftle, exp_pos = O.run_window_linear(vat, x0, 0.0, 0.05, 40, "rk4")
+    f = _field_2d(vels, pts, (96, 48), precision, pair=True)
+    pos = torch.as_tensor(x0, device="cuda")
+    out = torch.empty_like(pos)
+    ftle = W.advect_ftle(f, pos, method="rk4", time_mode="linear", n_steps=40, h=0.05,
+                         t_start=0.0, map_period=2.0, positions_out=out)
+    if precision == "float64":
+        np.testing.assert_array_equal(out.cpu().numpy(), exp_pos)
+        np.testing.assert_allclose(ftle.cpu().numpy(), exp_ftle, rtol=1e-12, atol=1e-14)
+    else:
+        err = _rel_range_err(ftle.cpu().numpy(), exp_ftle)
+        print(f"[fp32 linear 2D] max|err|/range = {err:.3e}")
+        assert err <= 1e-3
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_window_3d_periodic_abc(precision):
+    from pyftle_amd import _native as N
+    from pyftle_amd import window as W
+    from pyftle_amd.field import DeviceField, GridGeometry
+
+    n = 24
+    vels, pts = H.abc_window(n=n)
+    per = (True, True, True)
+    fields = H.grid_fields((n, n, n), vels, pts, periodic=per)
+    x0 = H.seeds_3d(5)
+    steps, h = 60, 0.05
+    exp_ftle, exp_pos = O.run_window_linear(lambda t: fields[0].interpolate, x0, 0.0, h, steps, "rk4")
+    geom = GridGeometry.from_reference((n, n, n), pts, periodic=per)
+    f = DeviceField(geom, dtype=precision, layout=N.LAYOUT_SINGLE, ring_slots=1, n_snapshots=1)
+    f.pack(0, vels[0])
+    pos = torch.as_tensor(x0, device="cuda")
+    out = torch.empty_like(pos)
+    ftle = W.advect_ftle(f, pos, method="rk4", time_mode="steady", n_steps=steps, h=h,
+                         map_period=steps * h, positions_out=out)
+    if precision == "float64":
+        np.testing.assert_array_equal(out.cpu().numpy(), exp_pos)
+        np.testing.assert_allclose(ftle.cpu().numpy(), exp_ftle, rtol=1e-10, atol=1e-13)
+    else:
+        err = _rel_range_err(ftle.cpu().numpy(), exp_ftle)
+        print(f"[fp32 steady 3D periodic] max|err|/range = {err:.3e}")
+        assert err <= 1e-3
+
+
+def test_ftle_solver_array_source_matches_oracle():
+    from pyftle_amd import NeighboringParticles
+    from pyftle_amd.data_source import ArrayBatchSource
+    from pyftle_amd.ftle_solver import FTLESolver
+    from pyftle_amd.integrate import create_integrator
+    from pyftle_amd.interpolate import create_interpolator
+
+    vels, pts = H.double_gyre_window(nx=64, ny=32, n_snap=11)
+    x0 = H.seeds_2d(16, 8)
+    fields = H.grid_fields((64, 32), vels, pts)
+    exp_ftle, exp_pos = O.run_window_reference([f.interpolate for f in fields], x0, 0.1, "rk4")
+    interp = create_interpolator("grid", (64, 32), precision="float64")
+    integ = create_integrator("RK4", interp)
+    parts = NeighboringParticles(x0.copy())
+    src = ArrayBatchSource(vels, pts, parts, 0.1)
+    ftle = FTLESolver(src, integ).run()
+    np.testing.assert_allclose(ftle, exp_ftle, rtol=1e-12, atol=1e-14)
+    np.testing.assert_array_equal(parts.positions, exp_pos)
+
+
+def test_ftle_solver_chunked_ring_matches_single_launch():
+    """A ring smaller than the window (several launches) gives the same field."""
+    from pyftle_amd import NeighboringParticles
+    from pyftle_amd.data_source import ArrayBatchSource
+    from pyftle_amd.ftle_solver import FTLESolver
+    from pyftle_amd.integrate import create_integrator
+    from pyftle_amd.interpolate import HighPerfInterpolator
+
+    vels, pts = H.double_gyre_window(nx=64, ny=32, n_snap=13)
+    x0 = H.seeds_2d(12, 6)
+    for method in ("rk4", "ab2", "euler"):
+        fields = H.grid_fields((64, 32), vels, pts)
+        exp_ftle, exp_pos = O.run_window_reference([f.interpolate for f in fields], x0, 0.1, method)
+        small = HighPerfInterpolator((64, 32), precision="float64",
+                                     max_ring_bytes=5 * 64 * 32 * 2 * 8)
+        parts = NeighboringParticles(x0.copy())
+        ftle = FTLESolver(ArrayBatchSource(vels, pts, parts, 0.1), create_integrator(method, small)).run()
+        np.testing.assert_allclose(ftle, exp_ftle, rtol=1e-12, atol=1e-14)
+        np.testing.assert_array_equal(parts.positions, exp_pos)
+
+
+@pytest.mark.parametrize("name", ["euler", "rk4", "ab2"])
+def test_per_step_integrators_match_oracle(name):
+    """Integrator.integrate(h, particles): one kernel step per call (integrate.py:161-342)."""
+    from pyftle_amd import NeighboringParticles
+    from pyftle_amd.integrate import create_integrator
+    from pyftle_amd.interpolate import create_interpolator
+
+    vels, pts = H.double_gyre_window(nx=64, ny=32, n_snap=4)
+    x0 = H.seeds_2d(8, 4)
+    interp = create_interpolator("grid", (64, 32), precision="float64")
+    integ = create_integrator(name, interp)
+    parts = NeighboringParticles(x0.copy())
+    xo = x0.copy()
+    v_prev = None
+    for i, v in enumerate(vels):
+        interp.update(v, pts if i == 0 else None)
+        integ.integrate(0.1, parts)
+        f = O.GridField((64, 32), v, pts).interpolate
+        if name == "rk4":
+            O.rk4_step(xo, 0.1, f)
+        elif name == "euler":
+            O.euler_step(xo, 0.1, f)
+        else:
+            v_prev = O.ab2_step(xo, 0.1, f, v_prev).copy()
+        np.testing.assert_array_equal(parts.positions, xo)
+
+
+def test_euler_kat_linear_field():
+    """tests/test_integrate.py:12-21: v = 0.1 x gives x + h*0.1*x (bilinear is exact)."""
+    from pyftle_amd import NeighboringParticles
+    from pyftle_amd.integrate import create_integrator
+    from pyftle_amd.interpolate import create_interpolator
+
+    x = np.linspace(0, 20, 21)
+    X, Y = np.meshgrid(x, x)
+    pts = np.stack([X.ravel(), Y.ravel()])
+    vel = 0.1 * pts
+    interp = create_interpolator("grid", (21, 21), precision="float64")
+    interp.update(vel, pts)
+    p0 = np.array([[1.0, 2.0], [3.0, 4.0], [5.0, 6.0], [7.0, 8.0],
+                   [9.0, 10.0], [11.0, 12.0], [13.0, 14.0], [15.0, 16.0]])
+    parts = NeighboringParticles(p0.copy())
+    create_integrator("euler", interp).integrate(0.1, parts)
+    np.testing.assert_allclose(parts.positions, p0 + 0.1 * p0 * 0.1, rtol=1e-14)
+    parts = NeighboringParticles(p0.copy())
+    create_integrator("rk4", interp).integrate(0.1, parts)
+    z = 0.01
+    np.testing.assert_allclose(parts.positions, p0 * (1 + z + z**2 / 2 + z**3 / 6 + z**4 / 24), rtol=1e-14)
