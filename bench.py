@@ -1,0 +1,384 @@
+#!/usr/bin/env python
+"""FTLE throughput benchmark on one MI355X.
+
+A step is one FTLE field: the fused advection kernel takes every seed group
+through the whole window and its FP64 epilogue writes the FTLE per seed.
+
+Workloads (synthetic, generated deterministically on the fly):
+* ``abc128`` (default): steady ABC flow (A=sqrt3, B=sqrt2, C=1) on a periodic
+  128^3 FP32 grid over [0, 2pi)^3; seeds on a 128^3 cell-centred grid, 6 axis
+  neighbours at delta=1e-3; RK4 h=0.01, 1000 steps, trilinear.
+* ``gyre512``: double gyre (A=0.1, eps=0.25, omega=2pi/10) on 512x256 over
+  [0,2]x[0,1], 101 snapshots t=0..10 (FP64 on host -> FP32 in HBM, PAIR
+  layout); 400x200 cell-centred seeds, 4 neighbours at 1e-3; RK4 h=0.05,
+  200 steps, linear in time.
+
+Prints one JSON line (metric, value = FTLE seeds/s, roofline of the advection
+kernel against 8 TB/s HBM, accuracy against FP64, CPU baseline).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FTLE seeds/s at fixed accuracy: fused RK4 vs MLP surrogate on 1 MI355X; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+
+def build_abc128(torch, n=128, ns=128, delta=1e-3, h=0.01, steps=1000):
+    import numpy as np
+
+    from pyftle_amd import _native as N
+    from pyftle_amd.field import DeviceField, GridGeometry
+
+    L = 2.0 * math.pi
+    sp = L / n
+    geom = GridGeometry.uniform((n, n, n), (0.0, 0.0, 0.0), (sp, sp, sp), periodic=(True, True, True))
+    field = DeviceField(geom, dtype="float32", layout=N.LAYOUT_SINGLE, ring_slots=1, n_snapshots=1)
+    ax = np.arange(n) * sp
+    Z, Y, X = np.meshgrid(ax, ax, ax, indexing="ij")  # flat ix + n*(iy + n*iz)
+    A, B, C = math.sqrt(3.0), math.sqrt(2.0), 1.0
+    vel = np.stack([A * np.sin(Z) + C * np.cos(Y), B * np.sin(X) + A * np.cos(Z),
+                    C * np.sin(Y) + B * np.cos(X)]).reshape(3, -1)
+    field.pack(0, vel)
+    dev = torch.device("cuda")
+    s = (torch.arange(ns, device=dev, dtype=torch.float64) + 0.5) * (L / ns)
+    cz, cy, cx = torch.meshgrid(s, s, s, indexing="ij")
+    centers = torch.stack([cx.reshape(-1), cy.reshape(-1), cz.reshape(-1)], dim=1)
+    blocks = []
+    for a in range(3):
+        e = torch.zeros(3, dtype=torch.float64, device=dev)
+        e[a] = delta
+        blocks += [centers - e, centers + e] if a == 0 else [centers + e, centers - e]
+    pos = torch.cat(blocks, dim=0).contiguous()
+    nseeds = ns ** 3
+    bytes_per_seed_step = 6 * 4 * 1 * 3 * 8 * 4  # nb * stages * snapshots * dim * 2^dim * 4 B
+    return dict(name="abc128", field=field, pos=pos, n_seeds=nseeds, steps=steps, h=h,
+                time_mode="steady", t_start=0.0, map_period=steps * h,
+                bytes_per_seed_step=bytes_per_seed_step, dim=3,
+                host=dict(vel=vel, n=n, sp=sp, periodic=True),
+                config={"workload": "abc128", "grid": [n, n, n], "periodic": True,
+                        "seeds": [ns, ns, ns], "neighbors": 6, "delta": delta, "integrator": "rk4",
+                        "h": h, "steps": steps, "interp": "trilinear", "field_dtype": "f32"})
+
+
+def _double_gyre(t, X, Y, A=0.1, eps=0.25, omega=2.0 * math.pi / 10.0):
+    import numpy as np
+
+    s = math.sin(omega * t)
+    f = eps * s * X**2 + (1.0 - 2.0 * eps * s) * X
+    dfdx = 2.0 * eps * s * X + (1.0 - 2.0 * eps * s)
+    u = -math.pi * A * np.sin(math.pi * f) * np.cos(math.pi * Y)
+    v = math.pi * A * np.cos(math.pi * f) * np.sin(math.pi * Y) * dfdx
+    return u, v
+
+
+def build_gyre(torch, nx=512, ny=256, n_snap=101, nsx=400, nsy=200, delta=1e-3, h=0.05, steps=200):
+    import numpy as np
+
+    from pyftle_amd import _native as N
+    from pyftle_amd.field import DeviceField, GridGeometry
+
+    geom = GridGeometry.uniform((nx, ny), (0.0, 0.0), (2.0 / (nx - 1), 1.0 / (ny - 1)))
+    field = DeviceField(geom, dtype="float32", layout=N.LAYOUT_PAIR, ring_slots=n_snap,
+                       n_snapshots=n_snap, t0=0.0, dt_snap=0.1)
+    x = np.linspace(0.0, 2.0, nx)
+    y = np.linspace(0.0, 1.0, ny)
+    X, Y = np.meshgrid(x, y)  # flat ix + nx*iy
+    snaps = []
+    for k in range(n_snap):
+        u, v = _double_gyre(0.1 * k, X, Y)
+        vel = np.stack([u.ravel(), v.ravel()])
+        field.pack(k, vel)
+        snaps.append(vel)
+    dev = torch.device("cuda")
+    xs = (torch.arange(nsx, device=dev, dtype=torch.float64) + 0.5) * (2.0 / nsx)
+    ys = (torch.arange(nsy, device=dev, dtype=torch.float64) + 0.5) * (1.0 / nsy)
+    cy, cx = torch.meshgrid(ys, xs, indexing="ij")
+    centers = torch.stack([cx.reshape(-1), cy.reshape(-1)], dim=1)
+    ex = torch.tensor([delta, 0.0], dtype=torch.float64, device=dev)
+    ey = torch.tensor([0.0, delta], dtype=torch.float64, device=dev)
+    pos = torch.cat([centers - ex, centers + ex, centers + ey, centers - ey], dim=0).contiguous()
+    bytes_per_seed_step = 4 * 4 * 2 * 2 * 4 * 4
+    return dict(name="gyre512", field=field, pos=pos, n_seeds=nsx * nsy, steps=steps, h=h,
+                time_mode="linear", t_start=0.0, map_period=steps * h,
+                bytes_per_seed_step=bytes_per_seed_step, dim=2,
+                host=dict(snaps=snaps, nx=nx, ny=ny),
+                config={"workload": "gyre512", "grid": [nx, ny], "snapshots": n_snap,
+                        "seeds": [nsx, nsy], "neighbors": 4, "delta": delta, "integrator": "rk4",
+                        "h": h, "steps": steps, "interp": "bilinear+linear-time", "field_dtype": "f32"})
+
+
+WORKLOADS = {"abc128": build_abc128, "gyre512": build_gyre}
+
+
+# ---------------------------------------------------------------------------
+# accuracy and CPU baseline (oracle used only as checker / baseline)
+# ---------------------------------------------------------------------------
+
+def accuracy_vs_fp64(torch, W, wl, ftle32, cfg):
+    """Same window in FP64 on the GPU (the kernel that matches the oracle bit
+    for bit on small cases), over the full field."""
+    from pyftle_amd.field import DeviceField
+
+    f32 = wl["field"]
+    f64 = DeviceField(f32.geom, dtype="float64", layout=f32.layout, ring_slots=f32.ring_slots,
+                      n_snapshots=f32.n_snapshots, t0=f32.t0, dt_snap=f32.dt_snap)
+    if wl["name"] == "abc128":
+        f64.pack(0, wl["host"]["vel"])
+    else:
+        for k, v in enumerate(wl["host"]["snaps"]):
+            f64.pack(k, v)
+    ftle64 = W.advect_ftle(f64, wl["pos"], method="rk4", time_mode=wl["time_mode"], n_steps=wl["steps"],
+                           h=wl["h"], t_start=wl["t_start"], map_period=wl["map_period"], cfg=cfg)
+    torch.cuda.synchronize()
+    a = ftle32.double()
+    b = ftle64
+    rng = (b.max() - b.min()).item()
+    err = (a - b).abs().max().item()
+    del f64
+    return {"reference": "fp64 fused kernel (bit-exact with the oracle on the parity cases)",
+            "max_abs_err": err, "field_range": rng, "max_abs_err_over_range": err / rng,
+            "rmse_over_range": ((a - b).pow(2).mean().sqrt().item()) / rng}
+
+
+def accuracy_vs_oracle(torch, wl, ftle32, n_sample=32, steps=None):
+    """FP64 NumPy oracle on a few seeds (the true checker)."""
+    import numpy as np
+
+    from oracle import numpy_oracle as O
+
+    pos = wl["pos"].cpu().numpy()
+    n = wl["n_seeds"]
+    nb = 2 * wl["dim"]
+    rng = np.random.default_rng(7)
+    idx = np.sort(rng.choice(n, size=n_sample, replace=False))
+    sub = np.concatenate([pos[j * n + idx] for j in range(nb)], axis=0)
+    if wl["name"] == "abc128":
+        h = wl["host"]
+        nn = h["n"]
+        comps = [h["vel"][c].reshape(nn, nn, nn).transpose(2, 1, 0) for c in range(3)]  # (x, y, z)
+        gf = O.GridField.__new__(O.GridField)
+        gf.dim, gf.v, gf.mins, gf.deltas, gf.periodic = 3, [np.ascontiguousarray(c) for c in comps], \
+            [0.0] * 3, [h["sp"]] * 3, (True, True, True)
+        exp, _ = O.run_window_linear(lambda t: gf.interpolate, sub, 0.0, wl["h"], wl["steps"], "rk4")
+    else:
+        h = wl["host"]
+        fields = []
+        for v in h["snaps"]:
+            gf = O.GridField.__new__(O.GridField)
+            gf.dim = 2
+            gf.v = [np.ascontiguousarray(v[c].reshape(h["ny"], h["nx"]).T) for c in range(2)]
+            gf.mins = [0.0, 0.0]
+            gf.deltas = [2.0 / (h["nx"] - 1), 1.0 / (h["ny"] - 1)]
+            gf.periodic = (False, False)
+            fields.append(gf)
+        vat = O.time_linear_velocity([f.interpolate for f in fields], 0.0, 0.1)
+        exp, _ = O.run_window_linear(vat, sub, 0.0, wl["h"], wl["steps"], "rk4")
+    got = ftle32.cpu().numpy()[idx]
+    return {"oracle_seeds": int(n_sample), "max_abs_err_vs_oracle": float(np.max(np.abs(got - exp)))}
+
+
+def cpu_baseline(wl, budget_s=12.0, n_seeds=2048):
+    """Reference CPU path (its own compiled GridInterp.cpp under a NumPy
+    restatement of the RK4 driver), single thread, random seed sample."""
+    import numpy as np
+
+    from oracle import native
+    from oracle.reference_path import ReferenceGridPath
+
+    if native.reference_ginterp() is None:
+        return None
+    pos = wl["pos"].cpu().numpy()
+    n = wl["n_seeds"]
+    nb = 2 * wl["dim"]
+    rng = np.random.default_rng(11)
+    idx = np.sort(rng.choice(n, size=min(n_seeds, n), replace=False))
+    sub = np.ascontiguousarray(np.concatenate([pos[j * n + idx] for j in range(nb)], axis=0))
+    h = wl["host"]
+    if wl["name"] == "abc128":
+        nn = h["n"]
+        comps = [h["vel"][c].reshape(nn, nn, nn).transpose(2, 1, 0) for c in range(3)]
+        path = ReferenceGridPath(comps, [0.0] * 3, [h["sp"]] * 3)
+    else:
+        v = h["snaps"][0]
+        comps = [v[c].reshape(h["ny"], h["nx"]).T for c in range(2)]
+        path = ReferenceGridPath(comps, [0.0, 0.0], [2.0 / (h["nx"] - 1), 1.0 / (h["ny"] - 1)])
+    done = 0
+    t0 = time.perf_counter()
+    while done < wl["steps"]:
+        path.rk4_steps(sub, wl["h"], 10)
+        done += 10
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    seed_steps = len(idx) * done / el
+    return {
+        "value": seed_steps / wl["steps"],
+        "unit": "seeds/s",
+        "cores": 1,
+        "kind": "reference",
+        "sample": f"{len(idx)} random seeds x {done} RK4 steps of the {wl['name']} window "
+                  f"({el:.1f} s); reference GridInterp.cpp compiled -O3, NumPy driver; "
+                  f"scaled to seeds/s of {wl['steps']}-step fields",
+        "seed_steps_per_s": seed_steps,
+        "cpu_model": _cpu_model(),
+        "host_nproc": os.cpu_count(),
+    }
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+# ---------------------------------------------------------------------------
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="abc128", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy check")
+    ap.add_argument("--groups-per-block", type=int, default=0)
+    ap.add_argument("--lane-order", type=int, default=-1)
+    ap.add_argument("--xcd-remap", type=int, default=-1)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only copy this many bytes device-to-device 3 times (counter calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every FTLE seed group is independent and this build "
+              "runs on a single MI355X; --gpus above 1 is not supported.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if args.copy_calibration:
+        n = args.copy_calibration // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4}), flush=True)
+        return
+
+    from pyftle_amd import _native
+    from pyftle_amd import window as W
+    from pyftle_amd.tuning import best_config, shape_class
+
+    _native.load()
+    torch.cuda.set_device(0)
+    log(f"building workload {args.workload}")
+    wl = WORKLOADS[args.workload](torch)
+    cfg = best_config(shape_class(wl["dim"], wl["n_seeds"], wl["time_mode"]))
+    if args.groups_per_block > 0:
+        cfg.groups_per_block = args.groups_per_block
+    if args.lane_order >= 0:
+        cfg.lane_order = args.lane_order
+    if args.xcd_remap >= 0:
+        cfg.xcd_remap = args.xcd_remap
+    ftle = torch.empty(wl["n_seeds"], dtype=torch.float64, device="cuda")
+
+    def launch():
+        W.advect_ftle(wl["field"], wl["pos"], method="rk4", time_mode=wl["time_mode"],
+                      n_steps=wl["steps"], h=wl["h"], t_start=wl["t_start"],
+                      map_period=wl["map_period"], ftle_out=ftle, cfg=cfg)
+
+    for _ in range(args.warmup):
+        launch()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()  # the stream the kernels are enqueued on
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record(stream)
+        launch()
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [a.elapsed_time(b) for a, b in evs]
+    avg_ms = sum(kernel_ms) / len(kernel_ms)
+    log(f"launches (ms): {', '.join(f'{x:.2f}' for x in kernel_ms)}; wall {wall:.3f} s")
+
+    n_seeds = wl["n_seeds"]
+    value = n_seeds * args.steps / wall
+    seed_steps = n_seeds * wl["steps"]
+    alg_bytes = wl["bytes_per_seed_step"] * seed_steps
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = load_traffic(wl["name"])
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic.get("hbm_bytes_per_launch") if traffic else None,
+            "kernel": "advect_kernel<3,RK4,STEADY,f32,FTLE>" if wl["dim"] == 3 else
+                      "advect_kernel<2,RK4,LINEAR,PAIR,f32,FTLE>",
+            "algorithmic_bytes_per_launch": alg_bytes,
+            "bytes_per_seed_step": wl["bytes_per_seed_step"],
+            "avg_launch_ms": avg_ms}
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "seeds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": wl["config"],
+        "seed_steps_per_s": seed_steps * args.steps / wall,
+        "roofline": roof,
+        "launch": {"groups_per_block": cfg.groups_per_block, "lane_order": cfg.lane_order,
+                   "xcd_remap": cfg.xcd_remap},
+    }
+    if not args.no_accuracy:
+        log("accuracy against FP64")
+        acc = accuracy_vs_fp64(torch, W, wl, ftle, cfg)
+        acc.update(accuracy_vs_oracle(torch, wl, ftle))
+        result["accuracy"] = acc
+    result["cpu_baseline"] = None
+    if not args.no_cpu:
+        log("CPU baseline")
+        result["cpu_baseline"] = cpu_baseline(wl)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

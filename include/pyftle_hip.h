@@ -52,14 +52,22 @@ extern "C" {
 #define PYFTLE_TIME_FROZEN 1 /* step i reads snapshot first_snapshot+i for all stages (ftle_solver.py:99-102) */
 #define PYFTLE_TIME_LINEAR 2 /* stage time t blends snapshots k,k+1 linearly (needs PYFTLE_LAYOUT_PAIR) */
 
+/* ---- compute precision of the advection (FP32 fields only; FP64 fields are FP64 throughout) ---- */
+#define PYFTLE_COMPUTE_NATIVE 0 /* FP32: compensated FP32 positions, FP32 index and blend */
+#define PYFTLE_COMPUTE_MIXED 1  /* FP64 positions, cell index and weights; FP32 corner blend */
+#define PYFTLE_COMPUTE_F64 2    /* FP64 positions and FP64 blend of the FP32 records */
+
 /* ---- work-to-lane order for grouped kernels ---- */
 #define PYFTLE_ORDER_SEED_MAJOR 0     /* the nb particles of a seed sit in adjacent lanes */
 #define PYFTLE_ORDER_NEIGHBOR_MAJOR 1 /* a wave holds one neighbour direction of consecutive seeds */
 
 /*
  * A gridded velocity field resident in HBM as a ring of snapshot slots.
- * Node (ix,iy[,iz]) of slot s is record  ix + n[0]*(iy + n[1]*iz)  at
- * data + s*slot_stride; snapshot (or pair) k lives in slot k % ring_slots.
+ * Node (ix,iy[,iz]) of slot s is record  ix + px*(iy + py*iz)  at
+ * data + s*slot_stride, with px = n[0] + periodic[0], py = n[1] + periodic[1]:
+ * a periodic axis stores one ghost node (a copy of node 0) at index n, written
+ * by pyftle_pack_snapshot.  Snapshot (or pair) k lives in slot k % ring_slots.
+ * FP32 slots are limited to 2 GiB; rows and xy-slices to < 2^24 records.
  * Physical -> index mapping is (p - origin)/spacing (grid_interp.py:78-79).
  */
 typedef struct pyftle_field {
@@ -89,6 +97,7 @@ typedef struct pyftle_advect_opts {
   int32_t lane_order;     /* PYFTLE_ORDER_* (grouped kernels) */
   int32_t groups_per_block; /* seeds per work-group (grouped kernels), 0 = default */
   int32_t xcd_remap;      /* 1: map consecutive work to one XCD's L2 (8 XCDs) */
+  int32_t compute;        /* PYFTLE_COMPUTE_* */
 } pyftle_advect_opts;
 
 /* ---- library ---- */

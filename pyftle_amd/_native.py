@@ -18,6 +18,7 @@ LAYOUT_SINGLE, LAYOUT_PAIR = 0, 1
 EULER, AB2, RK4 = 0, 1, 2
 TIME_STEADY, TIME_FROZEN, TIME_LINEAR = 0, 1, 2
 ORDER_SEED_MAJOR, ORDER_NEIGHBOR_MAJOR = 0, 1
+COMPUTE_NATIVE, COMPUTE_MIXED, COMPUTE_F64 = 0, 1, 2
 OK, EINVAL, EHIP = 0, -1, -2
 
 
@@ -55,6 +56,7 @@ class AdvectOpts(ctypes.Structure):
         ("lane_order", ctypes.c_int32),
         ("groups_per_block", ctypes.c_int32),
         ("xcd_remap", ctypes.c_int32),
+        ("compute", ctypes.c_int32),
     ]
 
 

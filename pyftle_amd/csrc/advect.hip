@@ -1,302 +1,10 @@
-// Fused advection kernel: Euler / AB2 / RK4 (integrate.py:16-109,161-342)
-// through a gridded velocity field (bi/trilinear in space, frozen or linear in
-// time), positions held in registers for the whole window, with an optional
-// FP64 FTLE epilogue (cauchy_green.py:15-153, ftle.py:9-233,
-// ftle_solver.py:122-161) so one launch turns initial seed groups into an
-// FTLE field.
-//
-// Work mapping: a work-group holds G seeds x nb particles (nb = 2*dim for the
-// FTLE window, nb = 1 for plain particle advection).  Lane order is
-// seed-major (the nb particles of a seed in adjacent lanes) or
-// neighbour-major (one neighbour direction of consecutive seeds per wave).
-//
-// FP32 compute keeps each coordinate as a compensated pair (hi, c) with
-// true = hi - c (Kahan summation of the per-step increments), initialised from
-// the FP64 input, so the neighbour separations the Jacobian needs do not drown
-// in the rounding of absolute positions.  FP64 compute follows the reference
-// operation order (stage x + (0.5*h)*k, update x += (h/6)*(k1+2k2+2k3+k4)).
+// C entry points of the fused advection kernel (kernel in advect_impl.hpp).
 #include <cmath>
 #include <string>
 
-#include "common.hpp"
+#include "advect_impl.hpp"
 
 namespace pyftle {
-
-struct AdvectArgs {
-  FieldDev f;
-  const void *pos_in;
-  void *pos_out;
-  void *vprev;
-  double *ftle_out;
-  int64_t n_groups;
-  int nb;
-  int G;
-  int lane_order;
-  int remap;
-  int pos_f64;
-  int n_steps;
-  int first;
-  int have_prev;
-  double t_start;
-  double h;
-  double map_period;
-};
-
-template <typename Real>
-struct PosState;
-
-template <>
-struct PosState<double> {
-  double x;
-  __device__ __forceinline__ void init(double v) { x = v; }
-  __device__ __forceinline__ double hi() const { return x; }
-  __device__ __forceinline__ void add(double inc) { x += inc; }
-  __device__ __forceinline__ double value() const { return x; }
-};
-
-template <>
-struct PosState<float> {
-  float x, c;  // true value = x - c
-  __device__ __forceinline__ void init(double v) {
-    x = (float)v;
-    c = (float)((double)x - v);
-  }
-  __device__ __forceinline__ float hi() const { return x; }
-  __device__ __forceinline__ void add(float inc) {
-    const float y = inc - c;
-    const float t = x + y;
-    c = (t - x) - y;
-    x = t;
-  }
-  __device__ __forceinline__ double value() const { return (double)x - (double)c; }
-};
-
-template <typename Real>
-__device__ __forceinline__ Real stage_pos(Real x, Real hh, Real k);
-template <>
-__device__ __forceinline__ double stage_pos<double>(double x, double hh, double k) {
-  return x + hh * k;  // integrate.py:327-338: tmp = x; tmp += (c*h) * k
-}
-template <>
-__device__ __forceinline__ float stage_pos<float>(float x, float hh, float k) {
-  return fmaf(hh, k, x);
-}
-
-template <typename Real>
-__device__ __forceinline__ Real rk4_increment(Real h6, Real k1, Real k2, Real k3, Real k4);
-template <>
-__device__ __forceinline__ double rk4_increment<double>(double h6, double k1, double k2, double k3,
-                                                        double k4) {
-  return h6 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);  // integrate.py:109
-}
-template <>
-__device__ __forceinline__ float rk4_increment<float>(float h6, float k1, float k2, float k3,
-                                                      float k4) {
-  return h6 * (fmaf(2.0f, k2 + k3, k1) + k4);
-}
-
-template <typename Real>
-__device__ __forceinline__ Real ab2_increment(Real h, Real v, Real vp) {
-  return h * ((Real)1.5 * v - (Real)0.5 * vp);  // integrate.py:78
-}
-
-template <int DIM, int METHOD, int TMODE, bool PAIR, typename Real, bool FTLE>
-__global__ void __launch_bounds__(512) advect_kernel(AdvectArgs a) {
-  const int64_t lb = logical_block(a.remap);
-  const int t = threadIdx.x;
-  const int nb = a.nb, G = a.G;
-  int sl, j;
-  if (a.lane_order == PYFTLE_ORDER_SEED_MAJOR) {
-    sl = t / nb;
-    j = t - sl * nb;
-  } else {
-    j = t / G;
-    sl = t - j * G;
-  }
-  const int64_t s = lb * G + sl;
-  const bool active = (t < G * nb) && (s < a.n_groups);
-  const int64_t pidx = (int64_t)j * a.n_groups + s;
-
-  double x0[DIM];
-  PosState<Real> X[DIM];
-#pragma unroll
-  for (int c = 0; c < DIM; ++c) {
-    double v = 0.0;
-    if (active)
-      v = a.pos_f64 ? ((const double *)a.pos_in)[pidx * DIM + c]
-                    : (double)((const float *)a.pos_in)[pidx * DIM + c];
-    x0[c] = v;
-    X[c].init(v);
-  }
-
-  const FieldDev &f = a.f;
-  const Real h = (Real)a.h;
-  Real vprev[DIM];
-  bool have_prev = (METHOD == PYFTLE_AB2) && a.have_prev;
-  if (METHOD == PYFTLE_AB2) {
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-      Real v = (Real)0;
-      if (active && have_prev && a.vprev)
-        v = a.pos_f64 ? (Real)((const double *)a.vprev)[pidx * DIM + c]
-                      : (Real)((const float *)a.vprev)[pidx * DIM + c];
-      vprev[c] = v;
-    }
-  }
-
-  const char *steady_slot = slot_ptr(f, a.first);
-  for (int step = 0; step < a.n_steps; ++step) {
-    const char *s0 = steady_slot;
-    const char *s1 = steady_slot;
-    const char *s2 = steady_slot;
-    double w0 = 0.0, w1 = 0.0, w2 = 0.0;
-    if (TMODE == PYFTLE_TIME_FROZEN) {
-      s0 = s1 = s2 = slot_ptr(f, a.first + step);
-    } else if (TMODE == PYFTLE_TIME_LINEAR) {
-      const double tt = a.t_start + step * a.h;
-      int k;
-      time_linear_lookup(f, tt, k, w0);
-      s0 = slot_ptr(f, k);
-      if (METHOD == PYFTLE_RK4) {
-        time_linear_lookup(f, tt + 0.5 * a.h, k, w1);
-        s1 = slot_ptr(f, k);
-        time_linear_lookup(f, tt + a.h, k, w2);
-        s2 = slot_ptr(f, k);
-      }
-    }
-    Real xs[DIM], k1[DIM];
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) xs[c] = X[c].hi();
-    sample_field<DIM, PAIR>(f, s0, (Real)w0, xs, k1);
-    if (METHOD == PYFTLE_RK4) {
-      Real k2[DIM], k3[DIM], k4[DIM];
-      const Real hh = (Real)(0.5 * a.h);
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) xs[c] = stage_pos<Real>(X[c].hi(), hh, k1[c]);
-      sample_field<DIM, PAIR>(f, s1, (Real)w1, xs, k2);
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) xs[c] = stage_pos<Real>(X[c].hi(), hh, k2[c]);
-      sample_field<DIM, PAIR>(f, s1, (Real)w1, xs, k3);
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) xs[c] = stage_pos<Real>(X[c].hi(), h, k3[c]);
-      sample_field<DIM, PAIR>(f, s2, (Real)w2, xs, k4);
-      const Real h6 = (Real)(a.h / 6.0);
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) X[c].add(rk4_increment<Real>(h6, k1[c], k2[c], k3[c], k4[c]));
-    } else if (METHOD == PYFTLE_EULER) {
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) X[c].add(h * k1[c]);  // integrate.py:44
-    } else {
-      if (have_prev) {
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) X[c].add(ab2_increment<Real>(h, k1[c], vprev[c]));
-      } else {
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) X[c].add(h * k1[c]);  // first AB2 step is Euler
-      }
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) vprev[c] = k1[c];
-      have_prev = true;
-    }
-  }
-
-  if (active && a.pos_out) {
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-      if (a.pos_f64)
-        ((double *)a.pos_out)[pidx * DIM + c] = X[c].value();
-      else
-        ((float *)a.pos_out)[pidx * DIM + c] = (float)X[c].value();
-    }
-  }
-  if (METHOD == PYFTLE_AB2 && active && a.vprev && a.n_steps > 0) {
-#pragma unroll
-    for (int c = 0; c < DIM; ++c) {
-      if (a.pos_f64)
-        ((double *)a.vprev)[pidx * DIM + c] = (double)vprev[c];
-      else
-        ((float *)a.vprev)[pidx * DIM + c] = (float)vprev[c];
-    }
-  }
-
-  if constexpr (FTLE) {
-    // LDS exchange: final and initial positions of every particle of the
-    // block, then lane sl < G forms the Jacobian of seed sl in FP64.
-    __shared__ double sh_fin[512 * DIM];
-    __shared__ double sh_ini[512 * DIM];
-    const int li = sl * nb + j;
-    if (t < G * nb) {
-#pragma unroll
-      for (int c = 0; c < DIM; ++c) {
-        sh_fin[li * DIM + c] = X[c].value();
-        sh_ini[li * DIM + c] = x0[c];
-      }
-    }
-    __syncthreads();
-    if (t < G) {
-      const int64_t seed = lb * G + t;
-      if (seed < a.n_groups) {
-        double F[DIM * DIM];
-#pragma unroll
-        for (int c = 0; c < DIM; ++c) {
-          const int pp = t * nb + pair_plus(c);
-          const int pm = t * nb + pair_minus(c);
-          const double d0 = sh_ini[pp * DIM + c] - sh_ini[pm * DIM + c];
-#pragma unroll
-          for (int r = 0; r < DIM; ++r)
-            F[r * DIM + c] = (sh_fin[pp * DIM + r] - sh_fin[pm * DIM + r]) / d0;
-        }
-        a.ftle_out[seed] = ftle_of_jacobian<DIM>(F, a.map_period);
-      }
-    }
-  }
-}
-
-template <int DIM, int METHOD, int TMODE, bool PAIR, typename Real, bool FTLE>
-static void launch_one(const AdvectArgs &a, unsigned blocks, unsigned threads, hipStream_t st) {
-  advect_kernel<DIM, METHOD, TMODE, PAIR, Real, FTLE><<<blocks, threads, 0, st>>>(a);
-}
-
-template <int DIM, int METHOD, int TMODE, bool PAIR, bool FTLE>
-static void dispatch_real(const AdvectArgs &a, int f64, unsigned blocks, unsigned threads,
-                          hipStream_t st) {
-  if (f64)
-    launch_one<DIM, METHOD, TMODE, PAIR, double, FTLE>(a, blocks, threads, st);
-  else
-    launch_one<DIM, METHOD, TMODE, PAIR, float, FTLE>(a, blocks, threads, st);
-}
-
-template <int DIM, int METHOD, bool FTLE>
-static void dispatch_time(const AdvectArgs &a, int tmode, int pair, int f64, unsigned blocks,
-                          unsigned threads, hipStream_t st) {
-  if (tmode == PYFTLE_TIME_LINEAR) {
-    dispatch_real<DIM, METHOD, PYFTLE_TIME_LINEAR, true, FTLE>(a, f64, blocks, threads, st);
-  } else if (tmode == PYFTLE_TIME_FROZEN) {
-    if (pair)
-      dispatch_real<DIM, METHOD, PYFTLE_TIME_FROZEN, true, FTLE>(a, f64, blocks, threads, st);
-    else
-      dispatch_real<DIM, METHOD, PYFTLE_TIME_FROZEN, false, FTLE>(a, f64, blocks, threads, st);
-  } else {
-    if (pair)
-      dispatch_real<DIM, METHOD, PYFTLE_TIME_STEADY, true, FTLE>(a, f64, blocks, threads, st);
-    else
-      dispatch_real<DIM, METHOD, PYFTLE_TIME_STEADY, false, FTLE>(a, f64, blocks, threads, st);
-  }
-}
-
-template <bool FTLE>
-static void dispatch_all(const AdvectArgs &a, int dim, int method, int tmode, int pair, int f64,
-                         unsigned blocks, unsigned threads, hipStream_t st) {
-  if (dim == 2) {
-    if (method == PYFTLE_RK4) dispatch_time<2, PYFTLE_RK4, FTLE>(a, tmode, pair, f64, blocks, threads, st);
-    else if (method == PYFTLE_EULER) dispatch_time<2, PYFTLE_EULER, FTLE>(a, tmode, pair, f64, blocks, threads, st);
-    else dispatch_time<2, PYFTLE_AB2, FTLE>(a, tmode, pair, f64, blocks, threads, st);
-  } else {
-    if (method == PYFTLE_RK4) dispatch_time<3, PYFTLE_RK4, FTLE>(a, tmode, pair, f64, blocks, threads, st);
-    else if (method == PYFTLE_EULER) dispatch_time<3, PYFTLE_EULER, FTLE>(a, tmode, pair, f64, blocks, threads, st);
-    else dispatch_time<3, PYFTLE_AB2, FTLE>(a, tmode, pair, f64, blocks, threads, st);
-  }
-}
 
 static int check_opts(const pyftle_field *field, const pyftle_advect_opts *o) {
   if (!o) return fail("advect: opts is NULL");
@@ -321,7 +29,23 @@ static int check_opts(const pyftle_field *field, const pyftle_advect_opts *o) {
   }
   if (o->lane_order != PYFTLE_ORDER_SEED_MAJOR && o->lane_order != PYFTLE_ORDER_NEIGHBOR_MAJOR)
     return fail("advect: unknown lane order");
+  if (o->compute != PYFTLE_COMPUTE_NATIVE && o->compute != PYFTLE_COMPUTE_MIXED &&
+      o->compute != PYFTLE_COMPUTE_F64)
+    return fail("advect: unknown compute precision");
+  if (field->dtype == PYFTLE_F64 && o->compute != PYFTLE_COMPUTE_NATIVE)
+    return fail("advect: an FP64 field always computes in FP64 (compute must be NATIVE)");
   return PYFTLE_OK;
+}
+
+static int prec_of(const pyftle_advect_opts *o) {
+  return o->compute == PYFTLE_COMPUTE_MIXED ? PREC_MIXED
+         : o->compute == PYFTLE_COMPUTE_F64 ? PREC_F64C
+                                            : PREC_NATIVE;
+}
+
+static void dispatch(const pyftle_field *field, const AdvectArgs &a, const LaunchSpec &sp) {
+  if (field->dim == 2) launch_advect_2d(a, sp);
+  else launch_advect_3d(a, sp);
 }
 
 }  // namespace pyftle
@@ -360,9 +84,10 @@ int pyftle_advect(const pyftle_field *field, const pyftle_advect_opts *opts, voi
   a.map_period = 1.0;
   const int64_t blocks = (n_particles + a.G - 1) / a.G;
   if (blocks > INT32_MAX) return fail("advect: too many particles");
-  dispatch_all<false>(a, field->dim, opts->method, opts->time_mode,
-                      field->layout == PYFTLE_LAYOUT_PAIR, field->dtype == PYFTLE_F64,
-                      (unsigned)blocks, 256, as_stream(stream));
+  LaunchSpec sp{opts->method, opts->time_mode, field->layout == PYFTLE_LAYOUT_PAIR,
+                field->dtype == PYFTLE_F64, prec_of(opts), 0, (unsigned)blocks, 256u,
+                as_stream(stream)};
+  dispatch(field, a, sp);
   return check_launch("advect");
 }
 
@@ -403,9 +128,10 @@ int pyftle_advect_ftle(const pyftle_field *field, const pyftle_advect_opts *opts
   if (blocks > INT32_MAX) return fail("advect_ftle: too many seeds");
   int threads = G * nb;
   threads = ((threads + 63) / 64) * 64;
-  dispatch_all<true>(a, field->dim, opts->method, opts->time_mode,
-                     field->layout == PYFTLE_LAYOUT_PAIR, field->dtype == PYFTLE_F64,
-                     (unsigned)blocks, (unsigned)threads, as_stream(stream));
+  LaunchSpec sp{opts->method, opts->time_mode, field->layout == PYFTLE_LAYOUT_PAIR,
+                field->dtype == PYFTLE_F64, prec_of(opts), 1, (unsigned)blocks, (unsigned)threads,
+                as_stream(stream)};
+  dispatch(field, a, sp);
   return check_launch("advect_ftle");
 }
 

@@ -40,12 +40,16 @@ int validate_field(const pyftle_field *f) {
   if (f->ring_slots < 1) return fail("field.ring_slots must be >= 1");
   if (f->n_snapshots < 1) return fail("field.n_snapshots must be >= 1");
   if (!f->data) return fail("field.data is NULL");
-  int64_t nodes = (int64_t)f->n[0] * f->n[1] * (f->dim == 3 ? f->n[2] : 1);
+  int64_t nodes = padded_nodes(f);
   if (nodes > (int64_t)INT32_MAX / 2) return fail("field grid too large (> 2^30 nodes)");
+  if (f->n[0] + 1 >= (1 << 24) || (int64_t)(f->n[0] + 1) * (f->n[1] + 1) >= (1 << 24))
+    return fail("field rows / slices must hold fewer than 2^24 nodes");
   int64_t need = nodes * record_bytes(f->dim, f->layout, f->dtype);
   if (f->slot_stride < need) return fail("field.slot_stride smaller than one slot of records");
   if (f->slot_stride % 16 != 0) return fail("field.slot_stride must be a multiple of 16 bytes");
   if ((reinterpret_cast<uintptr_t>(f->data) % 32) != 0) return fail("field.data must be 32-byte aligned");
+  if (f->dtype == PYFTLE_F32 && f->slot_stride > (int64_t)INT32_MAX)
+    return fail("FP32 field slots are limited to 2 GiB (32-bit buffer offsets)");
   return PYFTLE_OK;
 }
 
@@ -62,11 +66,20 @@ int make_field_dev(const pyftle_field *f, FieldDev *d) {
     d->periodic[a] = used ? (f->periodic[a] ? 1 : 0) : 0;
     d->origin[a] = used ? f->origin[a] : 0.0;
     d->spacing[a] = used ? f->spacing[a] : 1.0;
+    d->inv_spacing[a] = 1.0 / d->spacing[a];
+    d->n_d[a] = (double)d->n[a];
+    d->inv_n_d[a] = 1.0 / d->n_d[a];
     d->origin_f[a] = (float)d->origin[a];
     d->inv_spacing_f[a] = (float)(1.0 / d->spacing[a]);
     d->n_f[a] = (float)d->n[a];
     d->inv_n_f[a] = (float)(1.0 / d->n[a]);
   }
+  for (int a = 0; a < 3; ++a) {
+    d->per_n[a] = d->periodic[a] ? d->n[a] : 0;
+    d->lim[a] = d->periodic[a] ? d->n[a] : d->n[a] - 1;
+  }
+  d->px = d->n[0] + d->periodic[0];
+  d->pxy = d->px * (d->n[1] + d->periodic[1]);
   d->t0 = f->t0;
   d->dts = f->dt_snap;
   return PYFTLE_OK;

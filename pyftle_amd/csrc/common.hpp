@@ -34,8 +34,14 @@ struct FieldDev {
   int nsnap;
   int n[3];
   int periodic[3];
+  int px, pxy;     // record pitch of a row / an xy-slice (ghost node on periodic axes)
+  int per_n[3];    // n on periodic axes, 0 otherwise
+  int lim[3];      // valid lower-index bound: n (periodic) or n - 1 (zero outside)
   double origin[3];
   double spacing[3];
+  double inv_spacing[3];
+  double n_d[3];
+  double inv_n_d[3];
   float origin_f[3];
   float inv_spacing_f[3];
   float n_f[3];
@@ -45,6 +51,13 @@ struct FieldDev {
 
 int make_field_dev(const pyftle_field *f, FieldDev *out);
 int validate_field(const pyftle_field *f);
+
+// nodes of one slot including the ghost node of every periodic axis
+inline int64_t padded_nodes(const pyftle_field *f) {
+  int64_t g = 1;
+  for (int a = 0; a < f->dim; ++a) g *= (int64_t)f->n[a] + (f->periodic[a] ? 1 : 0);
+  return g;
+}
 
 // record size in bytes for (dim, layout, dtype)
 inline int record_bytes(int dim, int layout, int dtype) {
@@ -81,7 +94,8 @@ struct AxisIdx {
   bool ok;
 };
 
-__device__ __forceinline__ AxisIdx<double> axis_index(const FieldDev &f, int a, double p) {
+// FP64, reference semantics: division, integer modulo on periodic axes.
+__device__ __forceinline__ AxisIdx<double> axis_index_exact(const FieldDev &f, int a, double p) {
   AxisIdx<double> r;
   const double fi = (p - f.origin[a]) / f.spacing[a];
   const double fl = floor(fi);
@@ -99,6 +113,34 @@ __device__ __forceinline__ AxisIdx<double> axis_index(const FieldDev &f, int a, 
     r.i0 = i0;
     r.i1 = (i0 + 1 == n) ? 0 : i0 + 1;
   } else {
+    r.ok = finite_in_range && (i0 >= 0) && (i0 + 1 < n);
+    r.i0 = r.ok ? i0 : 0;
+    r.i1 = r.ok ? i0 + 1 : 0;
+  }
+  return r;
+}
+
+// FP64 positions of an FP32 field: multiply by the reciprocal spacing and wrap
+// periodic axes in floating point (fl - n*floor(fl/n) is exact for integral fl).
+__device__ __forceinline__ AxisIdx<double> axis_index(const FieldDev &f, int a, double p) {
+  AxisIdx<double> r;
+  const double fi = (p - f.origin[a]) * f.inv_spacing[a];
+  const double fl = floor(fi);
+  r.d = fi - fl;
+  const bool finite_in_range = (fl > -1.0e15) && (fl < 1.0e15);
+  const int n = f.n[a];
+  if (f.periodic[a]) {
+    const double q = floor(fl * f.inv_n_d[a]);
+    int w = finite_in_range ? (int)fma(-q, f.n_d[a], fl) : 0;
+    w = (w >= n) ? w - n : w;
+    w = (w < 0) ? w + n : w;
+    w = w < 0 ? 0 : (w >= n ? n - 1 : w);
+    r.ok = finite_in_range;
+    r.i0 = w;
+    r.i1 = (w + 1 == n) ? 0 : w + 1;
+  } else {
+    const double cl = fl < -1.0 ? -1.0 : (fl > 2147483000.0 ? 2147483000.0 : fl);
+    const int i0 = finite_in_range ? (int)cl : -1;
     r.ok = finite_in_range && (i0 >= 0) && (i0 + 1 < n);
     r.i0 = r.ok ? i0 : 0;
     r.i1 = r.ok ? i0 + 1 : 0;
@@ -140,6 +182,10 @@ template <>
 __device__ __forceinline__ float lerp_f<float>(float a, float b, float t) {
   return fmaf(t, b - a, a);
 }
+template <>
+__device__ __forceinline__ double lerp_f<double>(double a, double b, double t) {
+  return fma(t, b - a, a);
+}
 
 // ---------------------------------------------------------------------------
 // Sample the velocity at x from one slot.  w = time weight (PAIR layout).
@@ -158,14 +204,14 @@ __device__ __forceinline__ void sample_field(const FieldDev &f, const char *slot
   bool ok = true;
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
-    ax[a] = axis_index(f, a, x[a]);
+    ax[a] = axis_index_exact(f, a, x[a]);
     ok = ok && ax[a].ok;
   }
   if constexpr (DIM == 2) {
-    const int64_t r00 = (int64_t)ax[1].i0 * f.n[0] + ax[0].i0;
-    const int64_t r10 = (int64_t)ax[1].i0 * f.n[0] + ax[0].i1;
-    const int64_t r01 = (int64_t)ax[1].i1 * f.n[0] + ax[0].i0;
-    const int64_t r11 = (int64_t)ax[1].i1 * f.n[0] + ax[0].i1;
+    const int64_t r00 = (int64_t)ax[1].i0 * f.px + ax[0].i0;
+    const int64_t r10 = (int64_t)ax[1].i0 * f.px + ax[0].i1;
+    const int64_t r01 = (int64_t)ax[1].i1 * f.px + ax[0].i0;
+    const int64_t r11 = (int64_t)ax[1].i1 * f.px + ax[0].i1;
     double c[4][C];
     const int64_t idx[4] = {r00, r10, r01, r11};
 #pragma unroll
@@ -194,7 +240,7 @@ __device__ __forceinline__ void sample_field(const FieldDev &f, const char *slot
       vel[comp] = ok ? v : 0.0;
     }
   } else {
-    const int64_t nx = f.n[0], nxy = (int64_t)f.n[0] * f.n[1];
+    const int64_t nx = f.px, nxy = f.pxy;
     double lev[2][3];
     const double xd = ax[0].d, yd = ax[1].d, zd = ax[2].d;
     // corners: (x0/x1, y0/y1, z0/z1)
@@ -235,15 +281,18 @@ __device__ __forceinline__ void sample_field(const FieldDev &f, const char *slot
   }
 }
 
-template <int DIM, bool PAIR>
-__device__ __forceinline__ void sample_field(const FieldDev &f, const char *slot, float w,
-                                             const float *x, float *vel) {
+// FP32-stored field, any position / blend precision:
+//   Pos   = precision of the position, index and fractional weights;
+//   Blend = precision of the corner blend (lerps with fused multiply-add).
+template <int DIM, bool PAIR, typename Pos, typename Blend>
+__device__ __forceinline__ void sample_field_f32(const FieldDev &f, const char *slot, Blend w,
+                                                 const Pos *x, Blend *vel) {
   constexpr int C = RecInfo<DIM, PAIR>::comps;
   constexpr int L = RecInfo<DIM, PAIR>::per_level;
   using Rec = RecN<float, (C > 4 ? 4 : C)>;  // loads of <= 16 bytes
   constexpr int chunks = (C > 4) ? 2 : 1;
   const Rec *rec = reinterpret_cast<const Rec *>(slot);
-  AxisIdx<float> ax[DIM];
+  AxisIdx<Pos> ax[DIM];
   bool ok = true;
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
@@ -252,8 +301,8 @@ __device__ __forceinline__ void sample_field(const FieldDev &f, const char *slot
   }
   constexpr int NQ = (DIM == 2) ? 4 : 8;
   float c[NQ][C];
-  const int nx = f.n[0];
-  const int nxy = f.n[0] * f.n[1];
+  const int nx = f.px;
+  const int nxy = f.pxy;
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     const int ix = (q & 1) ? ax[0].i1 : ax[0].i0;
@@ -267,32 +316,38 @@ __device__ __forceinline__ void sample_field(const FieldDev &f, const char *slot
       for (int e = 0; e < (C > 4 ? 4 : C); ++e) c[q][h * 4 + e] = r.v[e];
     }
   }
-  float val[L];
+  const Blend xd = (Blend)ax[0].d, yd = (Blend)ax[1].d;
+  const Blend zd = (Blend)ax[2 % DIM].d;
+  Blend val[L];
 #pragma unroll
   for (int comp = 0; comp < DIM; ++comp) {
-    float lev[2];
+    Blend lev[2];
 #pragma unroll
     for (int l = 0; l < (PAIR ? 2 : 1); ++l) {
       const int e = l * L + comp;
-      const float xd = ax[0].d, yd = ax[1].d;
-      const float c0 = lerp_f(c[0][e], c[1][e], xd);
-      const float c1 = lerp_f(c[2][e], c[3][e], xd);
-      float r = lerp_f(c0, c1, yd);
+      const Blend c0 = lerp_f<Blend>(c[0][e], c[1][e], xd);
+      const Blend c1 = lerp_f<Blend>(c[2][e], c[3][e], xd);
+      Blend r = lerp_f<Blend>(c0, c1, yd);
       if constexpr (DIM == 3) {
-        const float zd = ax[2 % DIM].d;
-        const float c2 = lerp_f(c[4 % NQ][e], c[5 % NQ][e], xd);
-        const float c3 = lerp_f(c[6 % NQ][e], c[7 % NQ][e], xd);
-        const float r1 = lerp_f(c2, c3, yd);
-        r = lerp_f(r, r1, zd);
+        const Blend c2 = lerp_f<Blend>(c[4 % NQ][e], c[5 % NQ][e], xd);
+        const Blend c3 = lerp_f<Blend>(c[6 % NQ][e], c[7 % NQ][e], xd);
+        const Blend r1 = lerp_f<Blend>(c2, c3, yd);
+        r = lerp_f<Blend>(r, r1, zd);
       }
       lev[l] = r;
     }
-    float v = lev[0];
-    if (PAIR) v = lerp_f(lev[0], lev[1 % (PAIR ? 2 : 1)], w);
+    Blend v = lev[0];
+    if (PAIR) v = lerp_f<Blend>(lev[0], lev[1 % (PAIR ? 2 : 1)], w);
     val[comp] = v;
   }
 #pragma unroll
-  for (int comp = 0; comp < DIM; ++comp) vel[comp] = ok ? val[comp] : 0.0f;
+  for (int comp = 0; comp < DIM; ++comp) vel[comp] = ok ? val[comp] : (Blend)0;
+}
+
+template <int DIM, bool PAIR>
+__device__ __forceinline__ void sample_field(const FieldDev &f, const char *slot, float w,
+                                             const float *x, float *vel) {
+  sample_field_f32<DIM, PAIR, float, float>(f, slot, w, x, vel);
 }
 
 // ---------------------------------------------------------------------------

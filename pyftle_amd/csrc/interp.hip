@@ -65,21 +65,26 @@ __global__ void __launch_bounds__(256) interp3d_vec_kernel(const double *__restr
   out[i] = c;
 }
 
-// Snapshot ingest: one lane per grid node, reads the component-planar source
-// through element strides, writes one time level of the interleaved record.
+// Snapshot ingest: one lane per (padded) grid node, reads the component-planar
+// source through element strides, writes one time level of the interleaved
+// record.  A periodic axis carries one ghost node at index n holding node 0,
+// so an upper cell corner is always lower index + 1.
 template <typename Dst, typename Src>
 __global__ void __launch_bounds__(256) pack_kernel(char *__restrict__ slot_a, int level_a,
                                                    char *__restrict__ slot_b, int level_b,
                                                    const Src *__restrict__ src, int64_t s0,
                                                    int64_t s1, int64_t s2, int64_t s3, int dim,
-                                                   int nx, int ny, int nz, int per_level,
-                                                   int comps) {
+                                                   int nx, int ny, int nz, int gx, int gy, int gz,
+                                                   int per_level, int comps) {
   const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nodes = (int64_t)nx * ny * nz;
+  const int64_t nodes = (int64_t)gx * gy * gz;
   if (node >= nodes) return;
-  const int ix = (int)(node % nx);
-  const int iy = (int)((node / nx) % ny);
-  const int iz = (int)(node / ((int64_t)nx * ny));
+  int ix = (int)(node % gx);
+  int iy = (int)((node / gx) % gy);
+  int iz = (int)(node / ((int64_t)gx * gy));
+  ix = ix >= nx ? ix - nx : ix;
+  iy = iy >= ny ? iy - ny : iy;
+  iz = iz >= nz ? iz - nz : iz;
   const int64_t base = (int64_t)ix * s1 + (int64_t)iy * s2 + (int64_t)iz * s3;
   Dst vals[4];
 #pragma unroll
@@ -184,27 +189,30 @@ int pyftle_pack_snapshot(const pyftle_field *field, int32_t snapshot, const void
       }
     }
   }
-  const int64_t nodes = (int64_t)field->n[0] * field->n[1] * nz;
+  const int gx = field->n[0] + (field->periodic[0] ? 1 : 0);
+  const int gy = field->n[1] + (field->periodic[1] ? 1 : 0);
+  const int gz = nz + (dim == 3 && field->periodic[2] ? 1 : 0);
+  const int64_t nodes = (int64_t)gx * gy * gz;
   hipStream_t st = as_stream(stream);
   auto launch = [&](char *sa, int lva, char *sb, int lvb) {
     if (field->dtype == PYFTLE_F32) {
       if (src_dtype == PYFTLE_F64)
         pack_kernel<float, double><<<blocks_for(nodes, 256), 256, 0, st>>>(
             sa, lva, sb, lvb, (const double *)src, src_strides[0], src_strides[1], src_strides[2],
-            src_strides[3], dim, field->n[0], field->n[1], nz, per_level, comps);
+            src_strides[3], dim, field->n[0], field->n[1], nz, gx, gy, gz, per_level, comps);
       else
         pack_kernel<float, float><<<blocks_for(nodes, 256), 256, 0, st>>>(
             sa, lva, sb, lvb, (const float *)src, src_strides[0], src_strides[1], src_strides[2],
-            src_strides[3], dim, field->n[0], field->n[1], nz, per_level, comps);
+            src_strides[3], dim, field->n[0], field->n[1], nz, gx, gy, gz, per_level, comps);
     } else {
       if (src_dtype == PYFTLE_F64)
         pack_kernel<double, double><<<blocks_for(nodes, 256), 256, 0, st>>>(
             sa, lva, sb, lvb, (const double *)src, src_strides[0], src_strides[1], src_strides[2],
-            src_strides[3], dim, field->n[0], field->n[1], nz, per_level, comps);
+            src_strides[3], dim, field->n[0], field->n[1], nz, gx, gy, gz, per_level, comps);
       else
         pack_kernel<double, float><<<blocks_for(nodes, 256), 256, 0, st>>>(
             sa, lva, sb, lvb, (const float *)src, src_strides[0], src_strides[1], src_strides[2],
-            src_strides[3], dim, field->n[0], field->n[1], nz, per_level, comps);
+            src_strides[3], dim, field->n[0], field->n[1], nz, gx, gy, gz, per_level, comps);
     }
   };
   launch(a, la, b, lb);

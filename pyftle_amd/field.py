@@ -154,7 +154,8 @@ class DeviceField:
         dim = geom.dim
         comps = (2 if dim == 2 else 4) * (2 if layout == N.LAYOUT_PAIR else 1)
         esize = 4 if self.dtype == torch.float32 else 8
-        nodes = int(np.prod(geom.n))
+        # a periodic axis stores one ghost node (a copy of node 0) at index n
+        nodes = int(np.prod([geom.n[a] + (1 if geom.periodic[a] else 0) for a in range(dim)]))
         self.record_bytes = comps * esize
         self.slot_stride = -(-nodes * self.record_bytes // ALIGN) * ALIGN
         total = self.slot_stride * self.ring_slots + ALIGN

@@ -33,6 +33,7 @@ class LaunchConfig:
     groups_per_block: int = 64
     lane_order: int = N.ORDER_SEED_MAJOR
     xcd_remap: int = 1
+    compute: int = N.COMPUTE_NATIVE
 
 
 def make_opts(method, time_mode, n_steps, h, first=0, t_start=0.0, have_prev=False,
@@ -49,6 +50,7 @@ def make_opts(method, time_mode, n_steps, h, first=0, t_start=0.0, have_prev=Fal
     o.lane_order = int(cfg.lane_order)
     o.groups_per_block = int(cfg.groups_per_block)
     o.xcd_remap = int(cfg.xcd_remap)
+    o.compute = int(cfg.compute)
     return o
 
 
