@@ -1,0 +1,185 @@
+"""Host-side logic of the drop-in API (no GPU): factories, layouts, geometry, readers."""
+
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import flows
+from oracle import numpy_oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_integrator_factory_names_and_errors():
+    """integrate.py:350-387 (tests/test_integrate.py:40-63)."""
+    from pyftle_amd.integrate import (AdamsBashforth2Integrator, EulerIntegrator,
+                                      RungeKutta4Integrator, create_integrator)
+    from pyftle_amd.interpolate import HighPerfInterpolator
+
+    interp = HighPerfInterpolator((4, 4))
+    assert isinstance(create_integrator("ab2", interp), AdamsBashforth2Integrator)
+    assert isinstance(create_integrator("EULER", interp), EulerIntegrator)
+    assert isinstance(create_integrator("rK4", interp), RungeKutta4Integrator)
+    with pytest.raises(ValueError, match="Invalid integrator name 'invalid'.*"):
+        create_integrator("invalid", interp)
+    with pytest.raises(ValueError, match="Invalid integrator name ''.*"):
+        create_integrator("", interp)
+
+
+def test_interpolator_factory():
+    from pyftle_amd.interpolate import (AnalyticalInterpolator, ExecutableNotFoundError,
+                                        HighPerfInterpolator, create_interpolator)
+
+    assert isinstance(create_interpolator("GRID", (8, 8)), HighPerfInterpolator)
+    assert isinstance(create_interpolator("analytical", velocity_fn=lambda t, x: x), AnalyticalInterpolator)
+    with pytest.raises(ExecutableNotFoundError):
+        create_interpolator("analytical")
+    with pytest.raises(ValueError, match="Invalid interpolation type"):
+        create_interpolator("spline")
+    with pytest.raises(NotImplementedError):
+        create_interpolator("cubic", (8, 8))
+    with pytest.raises(ValueError, match="Unsupported grid dimensionality"):
+        HighPerfInterpolator((8,))
+
+
+def test_analytical_interpolator_time_is_frozen_at_zero():
+    """interpolate.py:788,812: velocity_fn(0.0, points) whatever the step."""
+    from pyftle_amd.interpolate import AnalyticalInterpolator
+
+    seen = []
+    ai = AnalyticalInterpolator(lambda t, x: (seen.append(t), x)[1])
+    ai.update(None)
+    ai.interpolate(np.zeros((3, 2)))
+    assert seen == [0.0]
+
+
+def test_neighboring_particles_reference_fixture():
+    """tests/test_neighboring_particles.py:8-162 values."""
+    from pyftle_amd import NeighboringParticles
+
+    p = NeighboringParticles(np.array([[0.0, 0.0], [1.0, 0.0], [0.5, 1.0], [0.5, 0.0]]))
+    assert len(p) == 1 and p.num_neighbors == 4
+    np.testing.assert_array_equal(p.initial_delta_right_left, [[1.0, 0.0]])
+    np.testing.assert_array_equal(p.initial_delta_top_bottom, [[0.0, 1.0]])
+    np.testing.assert_array_equal(p.initial_centroid, [[0.5, 0.25]])
+    p.positions[1] = [2.0, 0.0]
+    np.testing.assert_array_equal(p.delta_right_left, [[2.0, 0.0]])
+    np.testing.assert_array_equal(p.initial_delta_right_left, [[1.0, 0.0]])
+    p3 = NeighboringParticles(np.array([[0, 0, 0], [1, 0, 0], [0.5, 1, 0], [0.5, -1, 0],
+                                        [0.5, 0, 1], [0.5, 0, -1.0]]))
+    np.testing.assert_array_equal(p3.initial_delta_front_back, [[0.0, 0.0, 2.0]])
+
+
+def test_neighbor_positions_match_oracle(rng):
+    from pyftle_amd.particles import neighbor_positions
+
+    c = rng.uniform(0, 1, (7, 3))
+    np.testing.assert_array_equal(neighbor_positions(c, 1e-3), O.make_neighbor_positions(c, 1e-3))
+
+
+def test_grid_geometry_matches_reference_layout():
+    from pyftle_amd.field import GridGeometry
+
+    x, y = np.linspace(0.0, 2.0, 9), np.linspace(0.0, 1.0, 5)
+    vel, pts = flows.sample_2d(flows.double_gyre, 0.3, x, y)
+    g = GridGeometry.from_reference((9, 5), pts)
+    gf = O.GridField((9, 5), vel, pts)
+    assert g.n == (9, 5) and list(g.origin) == gf.mins and list(g.spacing) == gf.deltas
+    # element (c, ix, iy) through the strides equals the oracle's component array
+    s0, s1, s2, _ = g.src_strides
+    for ix, iy in ((3, 2), (8, 4), (0, 1)):
+        assert vel.ravel()[1 * s0 + ix * s1 + iy * s2] == gf.v[1][ix, iy]
+    a = np.linspace(0, 1, 4)
+    vel3, pts3 = flows.sample_3d(lambda t, q: q * [1, 2, 3], 0.0, a, a, a)
+    g3 = GridGeometry.from_reference((4, 4, 4), pts3)
+    gf3 = O.GridField((4, 4, 4), vel3, pts3)
+    s0, s1, s2, s3 = g3.src_strides
+    for i, j, k in ((1, 2, 3), (3, 0, 1)):
+        for c in range(3):
+            assert vel3.ravel()[c * s0 + i * s1 + j * s2 + k * s3] == gf3.v[c][i, j, k]
+
+
+def test_file_readers_roundtrip(tmp_path):
+    """file_readers.py:35-182 keys and block concatenation order L, R, T, B(, F, Bk)."""
+    from scipy.io import savemat
+
+    from pyftle_amd.file_readers import read_coordinate, read_seed_particles_coordinates, read_velocity
+
+    savemat(tmp_path / "v.mat", {"velocity_x": np.arange(6.0).reshape(2, 3), "velocity_y": np.ones((2, 3))})
+    v = read_velocity(str(tmp_path / "v.mat"))
+    assert v.shape == (2, 6) and v[0, 4] == 4.0
+    savemat(tmp_path / "c.mat", {"coordinate_x": np.zeros(4), "coordinate_y": np.ones(4), "coordinate_z": 2 * np.ones(4)})
+    assert read_coordinate(str(tmp_path / "c.mat")).shape == (3, 4)
+    blocks = {k: np.full((2, 2), i, float) for i, k in enumerate(["left", "right", "top", "bottom"])}
+    savemat(tmp_path / "p.mat", blocks)
+    p = read_seed_particles_coordinates(str(tmp_path / "p.mat"))
+    np.testing.assert_array_equal(p.positions[:, 0], [0, 0, 1, 1, 2, 2, 3, 3])
+    savemat(tmp_path / "bad.mat", {"left": np.zeros((1, 2))})
+    with pytest.raises(ValueError):
+        read_seed_particles_coordinates(str(tmp_path / "bad.mat"))
+
+
+def test_file_batch_source_reuses_coordinates(tmp_path):
+    """data_source.py:140-211: coordinates read on step 0 only when one file is reused."""
+    from scipy.io import savemat
+
+    from pyftle_amd.data_source import FileBatchSource
+
+    savemat(tmp_path / "c.mat", {"coordinate_x": np.zeros(4), "coordinate_y": np.ones(4)})
+    for k in range(3):
+        savemat(tmp_path / f"v{k}.mat", {"velocity_x": np.zeros(4), "velocity_y": np.ones(4) * k})
+    src = FileBatchSource([str(tmp_path / f"v{k}.mat") for k in range(3)], [str(tmp_path / "c.mat")] * 3,
+                          "p.mat", 0.1, 0.2)
+    assert src.num_steps == 3 and src.timestep == 0.1 and src.id == "v0"
+    assert src.get_data_for_step(0)[1] is not None
+    assert src.get_data_for_step(2)[1] is None
+
+
+def test_analytical_solver_windows():
+    """api.py:105,125-155: p = int(T/|dt|)+1 snapshots, windows start at k*dt."""
+    from pyftle_amd import NeighboringParticles
+    from pyftle_amd.api import AnalyticalSolver
+
+    parts = NeighboringParticles(O.make_neighbor_positions(np.array([[1.0, 0.5]]), 1e-3))
+    s = AnalyticalSolver(flows.double_gyre, parts, 0.1, 0.3, 3, "rk4")
+    assert s.num_snapshots == int(0.3 / 0.1) + 1 == 3
+    b = s._create_batches()
+    assert [x.num_steps for x in b] == [3, 3, 3]
+    np.testing.assert_allclose(b[2].times, [0.2, 0.3, 0.4])
+    assert b[1].id == f"{0.1:06f}"
+    assert b[0].get_particles() is not parts
+
+
+def test_launch_config_to_opts():
+    from pyftle_amd import _native as N
+    from pyftle_amd import window as W
+
+    o = W.make_opts("ab2", "linear", 7, -0.05, first=3, t_start=1.5,
+                    cfg=W.LaunchConfig(groups_per_block=32, lane_order=1, xcd_remap=0))
+    assert (o.method, o.time_mode, o.n_steps, o.first_snapshot) == (N.AB2, N.TIME_LINEAR, 7, 3)
+    assert (o.h, o.t_start, o.groups_per_block, o.lane_order, o.xcd_remap) == (-0.05, 1.5, 32, 1, 0)
+
+
+def test_tuning_shape_classes():
+    from pyftle_amd import tuning
+
+    assert tuning.shape_class(3, 128**3, "steady") == "d3-steady-f32-s21"
+    cands = tuning.candidates(3)
+    assert all(c.groups_per_block * 6 <= 512 for c in cands)
+    assert tuning.best_config("no-such-class").groups_per_block == 64
+
+
+def test_library_missing_raises(tmp_path, monkeypatch):
+    from pyftle_amd import _native
+
+    with pytest.raises(RuntimeError, match="not built"):
+        _native.load(str(tmp_path / "nope.so"))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr

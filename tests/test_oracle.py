@@ -1,0 +1,219 @@
+"""Pins of the CPU oracle (no GPU): golden vectors, reference KATs, closed forms.
+
+Each test is chosen so that a plausible slip in the oracle (a dropped term, a
+wrong index or sign, a transposed operand, a wrong step count) fails it.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from oracle import flows
+from oracle import native as ON
+from oracle import numpy_oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "ginterp_vectors.npz")
+
+
+# ------------------------------------------------------------ interpolation --
+
+def test_interp_matches_reference_golden_vectors():
+    """Vectors written by the reference's own GridInterp.cpp (tests/golden)."""
+    g = np.load(GOLDEN)
+    np.testing.assert_array_equal(O.interp2d_vec(g["v2"], g["p2"]), g["o2"])
+    np.testing.assert_array_equal(O.interp3d_vec(g["v3"], g["p3"]), g["o3"])
+
+
+def test_interp_matches_compiled_reference(rng):
+    ref = ON.reference_ginterp()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    v = rng.standard_normal((23, 19))
+    p = rng.uniform(-2, 25, (4000, 2))
+    out = np.empty(len(p))
+    ref.interp2d_vec_inplace(v, p, out)
+    np.testing.assert_array_equal(O.interp2d_vec(v, p), out)
+    v3 = rng.standard_normal((7, 8, 9))
+    p3 = rng.uniform(-1, 10, (4000, 3))
+    out3 = np.empty(len(p3))
+    ref.interp3d_vec_inplace(v3, p3, out3)
+    np.testing.assert_array_equal(O.interp3d_vec(v3, p3), out3)
+
+
+def test_interp_exact_on_multilinear_fields(rng):
+    nx, ny, nz = 9, 7, 5
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    f2 = 0.3 + 1.7 * X - 0.9 * Y + 0.25 * X * Y
+    p = rng.uniform(0, [nx - 1 - 1e-9, ny - 1 - 1e-9], (500, 2))
+    exp = 0.3 + 1.7 * p[:, 0] - 0.9 * p[:, 1] + 0.25 * p[:, 0] * p[:, 1]
+    np.testing.assert_allclose(O.interp2d_vec(f2, p), exp, rtol=1e-12, atol=1e-12)
+    X, Y, Z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    f3 = 1.0 + X - 2 * Y + 3 * Z + 0.5 * X * Y * Z - 0.25 * Y * Z
+    p3 = rng.uniform(0, [nx - 1 - 1e-9, ny - 1 - 1e-9, nz - 1 - 1e-9], (500, 3))
+    x, y, z = p3.T
+    exp3 = 1.0 + x - 2 * y + 3 * z + 0.5 * x * y * z - 0.25 * y * z
+    np.testing.assert_allclose(O.interp3d_vec(f3, p3), exp3, rtol=1e-12, atol=1e-12)
+
+
+def test_interp_boundary_semantics():
+    """Zero outside, zero exactly on the upper edge (GridInterp.cpp:126), grid value at the lower edge."""
+    v = np.arange(12.0).reshape(4, 3) + 1.0
+    pts = np.array([[0, 0], [3, 0], [0, 2], [2.999999, 1.5], [-1e-12, 0], [1, 1], [np.nan, 0]])
+    out = O.interp2d_vec(v, pts)
+    assert out[0] == v[0, 0]
+    assert out[1] == 0.0 and out[2] == 0.0 and out[4] == 0.0 and out[6] == 0.0
+    assert out[3] != 0.0
+    assert out[5] == v[1, 1]
+
+
+def test_interp_matches_scipy_regular_grid(rng):
+    from scipy.interpolate import RegularGridInterpolator
+
+    v = rng.standard_normal((15, 11))
+    rgi = RegularGridInterpolator((np.arange(15.0), np.arange(11.0)), v, method="linear",
+                                  bounds_error=False, fill_value=0.0)
+    p = rng.uniform(-1, 15, (3000, 2))
+    inside_upper = (p[:, 0] == 14) | (p[:, 1] == 10)
+    got = O.interp2d_vec(v, p)
+    np.testing.assert_allclose(got[~inside_upper], rgi(p)[~inside_upper], rtol=1e-12, atol=1e-12)
+
+
+def test_c_port_matches_numpy_oracle(rng):
+    v = rng.standard_normal((13, 9))
+    p = rng.uniform(-1, 14, (2000, 2))
+    np.testing.assert_array_equal(ON.cpu_interp2d(v, p), O.interp2d_vec(v, p))
+    v3 = rng.standard_normal((6, 7, 5))
+    p3 = rng.uniform(-1, 8, (2000, 3))
+    np.testing.assert_array_equal(ON.cpu_interp3d(v3, p3), O.interp3d_vec(v3, p3))
+
+
+def test_grid_field_layouts_follow_reference():
+    """2D flat x-fastest (interpolate.py:363); 3D axis swap (interpolate.py:383-385)."""
+    nx, ny = 6, 4
+    x, y = np.linspace(0, 5, nx), np.linspace(0, 3, ny)
+    vel, pts = flows.sample_2d(lambda t, q: np.stack([q[:, 0] + 10 * q[:, 1], -q[:, 1]], 1), 0.0, x, y)
+    gf = O.GridField((nx, ny), vel, pts)
+    assert gf.v[0].shape == (nx, ny)
+    assert gf.v[0][2, 1] == x[2] + 10 * y[1]
+    assert gf.deltas == [1.0, 1.0] and gf.mins == [0.0, 0.0]
+    n = 5
+    a = np.linspace(0, 4, n)
+    vel3, pts3 = flows.sample_3d(lambda t, q: np.stack([q[:, 0], 2 * q[:, 1], 3 * q[:, 2]], 1), 0.0, a, a, a)
+    g3 = O.GridField((n, n, n), vel3, pts3)
+    assert g3.v[0][3, 1, 2] == a[3] and g3.v[1][3, 1, 2] == 2 * a[1] and g3.v[2][3, 1, 2] == 3 * a[2]
+
+
+# --------------------------------------------------------------- stepping --
+
+def _linear_field(a=0.1, n=41, L=20.0):
+    x = np.linspace(0, L, n)
+    X, Y = np.meshgrid(x, x)
+    pts = np.stack([X.ravel(), Y.ravel()])
+    return O.GridField((n, n), a * pts, pts)
+
+
+def test_euler_kat_reference():
+    """tests/test_integrate.py:12-21: v = 0.1 x -> x + h * 0.1 x."""
+    p0 = np.arange(1.0, 17.0).reshape(8, 2)
+    f = _linear_field()
+    x = p0.copy()
+    O.euler_step(x, 0.1, f.interpolate)
+    np.testing.assert_allclose(x, p0 + 0.1 * p0 * 0.1, rtol=1e-14)
+
+
+def test_rk4_stability_factor():
+    a, h = 0.1, 0.1
+    p0 = np.arange(1.0, 17.0).reshape(8, 2)
+    f = _linear_field(a)
+    x = p0.copy()
+    O.rk4_step(x, h, f.interpolate)
+    z = a * h
+    np.testing.assert_allclose(x, p0 * (1 + z + z**2 / 2 + z**3 / 6 + z**4 / 24), rtol=1e-14)
+
+
+def test_ab2_recurrence():
+    a, h = 0.1, 0.1
+    f = _linear_field(a)
+    x = np.array([[2.0, 3.0]] * 4)
+    v = O.ab2_step(x, h, f.interpolate, None)  # Euler first
+    np.testing.assert_allclose(x[0], [2.0 * (1 + a * h), 3.0 * (1 + a * h)], rtol=1e-14)
+    x1 = x.copy()
+    O.ab2_step(x, h, f.interpolate, v)
+    np.testing.assert_allclose(x[0], x1[0] + h * (1.5 * a * x1[0] - 0.5 * v[0]), rtol=1e-14)
+
+
+def test_c_port_advect_matches_oracle():
+    vels, pts = [], None
+    x, y = flows.grid_axes_2d(48, 24)
+    for k in range(6):
+        v, pts = flows.sample_2d(flows.double_gyre, 0.1 * k, x, y)
+        vels.append(v)
+    fields = [O.GridField((48, 24), v, pts) for v in vels]
+    x0 = O.make_neighbor_positions(flows.cell_centred_seeds_2d(10, 5), 1e-3)
+    for method in ("rk4", "euler", "ab2"):
+        _, exp = O.run_window_reference([f.interpolate for f in fields], x0, 0.1, method)
+        got = ON.cpu_advect(fields, x0, 0.0, 0.1, 6, method=method, mode=0)
+        np.testing.assert_array_equal(got, exp)
+
+
+# ------------------------------------------------- particles and FTLE --
+
+def test_particle_blocks_reference_values():
+    """tests/test_neighboring_particles.py fixtures: [L, R, T, B(, F, Bk)]."""
+    p2 = np.array([[0.0, 0.0], [1.0, 0.0], [0.5, 1.0], [0.5, 0.0]])
+    np.testing.assert_array_equal(O.delta_right_left(p2, 4), [[1.0, 0.0]])
+    np.testing.assert_array_equal(O.delta_top_bottom(p2, 4), [[0.0, 1.0]])
+    np.testing.assert_array_equal(O.centroid(p2, 4), [[0.5, 0.25]])
+    p3 = np.array([[0, 0, 0], [1, 0, 0], [0.5, 1, 0], [0.5, -1, 0], [0.5, 0, 1], [0.5, 0, -1.0]])
+    np.testing.assert_array_equal(O.delta_front_back(p3, 6), [[0.0, 0.0, 2.0]])
+    np.testing.assert_array_equal(O.delta_top_bottom(p3, 6), [[0.0, 2.0, 0.0]])
+
+
+def test_jacobian_of_linear_map(rng):
+    for dim in (2, 3):
+        M = rng.standard_normal((dim, dim))
+        x0 = O.make_neighbor_positions(rng.uniform(-1, 1, (20, dim)), 0.25)
+        J = O.flow_map_jacobian(x0 @ M.T, x0)
+        np.testing.assert_allclose(J, np.broadcast_to(M, J.shape), rtol=1e-12, atol=1e-12)
+
+
+def test_ftle_kats(jacobians_2x2, jacobians_3x3):
+    """tests/test_ftle.py:17-92."""
+    cg = O.cauchy_green_2x2(jacobians_2x2)
+    np.testing.assert_allclose(cg, np.einsum("...ij,...kj->...ik", jacobians_2x2, jacobians_2x2), rtol=1e-12)
+    np.testing.assert_allclose(O.max_eigenvalue_2x2(cg), [4.0, 4.0], rtol=1e-15)
+    np.testing.assert_allclose(O.ftle_2x2(jacobians_2x2, 1.0), [np.log(2)] * 2, rtol=1e-14)
+    cg3 = O.cauchy_green_3x3(jacobians_3x3)
+    np.testing.assert_allclose(O.max_eigenvalue_3x3(cg3), np.linalg.eigvalsh(cg3)[:, -1], rtol=1e-12)
+    np.testing.assert_allclose(O.ftle_3x3(np.repeat(np.eye(3)[None], 2, 0), 1.0), [0.0, 0.0], atol=1e-15)
+    # 2D C = F^T F, not F F^T: a non-symmetric F tells them apart in the off-diagonal
+    F = np.array([[[1.0, 2.0], [0.0, 1.0]]])
+    np.testing.assert_array_equal(O.cauchy_green_2x2(F), [[[1.0, 2.0], [2.0, 5.0]]])
+
+
+def test_window_arithmetic_saddle_closed_form():
+    """Reference window: p snapshots -> p steps, FTLE / ((p-1)|dt|) (ftle_solver.py:99,156,160).
+
+    u = l x, v = -l y is reproduced exactly by bilinear interpolation, so after
+    p RK4 steps the x separation grows by R(l h)^p and FTLE = p ln R(l h) / ((p-1) h).
+    """
+    lam, h, p = 0.5, 0.1, 11
+    x = np.linspace(-3, 3, 61)
+    vel, pts = flows.sample_2d(lambda t, q: flows.saddle(t, q, lam), 0.0, x, x)
+    f = O.GridField((61, 61), vel, pts)
+    x0 = O.make_neighbor_positions(np.array([[0.0, 0.5], [0.1, -0.3]]), 1e-3)
+    ftle, _ = O.run_window_reference([f.interpolate] * p, x0, h, "rk4")
+    z = lam * h
+    R = 1 + z + z**2 / 2 + z**3 / 6 + z**4 / 24
+    np.testing.assert_allclose(ftle, p * np.log(R) / ((p - 1) * h), rtol=1e-9)
+
+
+def test_time_linear_blend_exact_for_linear_in_time():
+    f0 = lambda q: np.ones_like(q)  # noqa: E731
+    f1 = lambda q: 3 * np.ones_like(q)  # noqa: E731
+    vat = O.time_linear_velocity([f0, f1], 0.0, 2.0)
+    q = np.zeros((3, 2))
+    np.testing.assert_allclose(vat(0.5)(q), 1.5)
+    np.testing.assert_allclose(vat(2.0)(q), 3.0)
+    np.testing.assert_allclose(vat(5.0)(q), 3.0)  # clamped
