@@ -125,7 +125,82 @@ def build_gyre(torch, nx=512, ny=256, n_snap=101, nsx=400, nsy=200, delta=1e-3, 
                         "h": h, "steps": steps, "interp": "bilinear+linear-time", "field_dtype": "f32"})
 
 
-WORKLOADS = {"abc128": build_abc128, "gyre512": build_gyre}
+def batch64_table(n_fields=64, seed=0):
+    """The sampled batch: eps~U[0.1,0.3], omega~U[0.5,1.5]*2pi/10, grid, seed grid
+    and window drawn with numpy.random.default_rng(seed) (committed as
+    tests/golden/batch64_table.json)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    grids = [(128, 64), (256, 128), (512, 256), (1024, 512)]
+    seedg = [(64, 32), (128, 64), (256, 128), (512, 256), (1024, 512), (2048, 1024)]
+    Ts = [5.0, 10.0, 15.0, 20.0]
+    out = []
+    for _ in range(n_fields):
+        eps = float(rng.uniform(0.1, 0.3))
+        omega = float(rng.uniform(0.5, 1.5) * 2.0 * math.pi / 10.0)
+        g = grids[int(rng.integers(len(grids)))]
+        sg = seedg[int(rng.integers(len(seedg)))]
+        T = Ts[int(rng.integers(len(Ts)))]
+        out.append({"eps": eps, "omega": omega, "grid": list(g), "seeds": list(sg), "T": T})
+    return out
+
+
+def _gyre_field_on_device(torch, nx, ny, n_snap, eps, omega, dt_snap=0.1, A=0.1, dtype="float32"):
+    from pyftle_amd import _native as N
+    from pyftle_amd.field import DeviceField, GridGeometry
+
+    geom = GridGeometry.uniform((nx, ny), (0.0, 0.0), (2.0 / (nx - 1), 1.0 / (ny - 1)))
+    field = DeviceField(geom, dtype=dtype, layout=N.LAYOUT_PAIR, ring_slots=n_snap,
+                        n_snapshots=n_snap, t0=0.0, dt_snap=dt_snap)
+    dev = torch.device("cuda")
+    x = torch.linspace(0.0, 2.0, nx, dtype=torch.float64, device=dev)
+    y = torch.linspace(0.0, 1.0, ny, dtype=torch.float64, device=dev)
+    Y, X = torch.meshgrid(y, x, indexing="ij")  # flat ix + nx*iy
+    for k in range(n_snap):
+        sn = math.sin(omega * dt_snap * k)
+        f = eps * sn * X * X + (1.0 - 2.0 * eps * sn) * X
+        dfdx = 2.0 * eps * sn * X + (1.0 - 2.0 * eps * sn)
+        u = -math.pi * A * torch.sin(math.pi * f) * torch.cos(math.pi * Y)
+        v = math.pi * A * torch.cos(math.pi * f) * torch.sin(math.pi * Y) * dfdx
+        field.pack(k, torch.stack([u.reshape(-1), v.reshape(-1)]))
+    return field
+
+
+def _gyre_seeds(torch, nsx, nsy, delta=1e-3):
+    dev = torch.device("cuda")
+    xs = (torch.arange(nsx, device=dev, dtype=torch.float64) + 0.5) * (2.0 / nsx)
+    ys = (torch.arange(nsy, device=dev, dtype=torch.float64) + 0.5) * (1.0 / nsy)
+    cy, cx = torch.meshgrid(ys, xs, indexing="ij")
+    c = torch.stack([cx.reshape(-1), cy.reshape(-1)], dim=1)
+    ex = torch.tensor([delta, 0.0], dtype=torch.float64, device=dev)
+    ey = torch.tensor([0.0, delta], dtype=torch.float64, device=dev)
+    return torch.cat([c - ex, c + ex, c + ey, c - ey], dim=0).contiguous()
+
+
+def build_batch64(torch, h=0.05, n_fields=64):
+    from pyftle_amd.batch import Window
+
+    table = batch64_table(n_fields)
+    windows = []
+    for row in table:
+        nx, ny = row["grid"]
+        n_snap = int(round(row["T"] / 0.1)) + 1
+        field = _gyre_field_on_device(torch, nx, ny, n_snap, row["eps"], row["omega"])
+        pos = _gyre_seeds(torch, *row["seeds"])
+        steps = int(round(row["T"] / h))
+        windows.append(Window(field=field, positions=pos, n_steps=steps, h=h, map_period=steps * h))
+    n_seeds = sum(w.positions.shape[0] // 4 for w in windows)
+    seed_steps = sum(w.positions.shape[0] // 4 * w.n_steps for w in windows)
+    return dict(name="batch64", windows=windows, n_seeds=n_seeds, seed_steps=seed_steps, dim=2,
+                bytes_per_seed_step=1024, table=table,
+                config={"workload": "batch64", "fields": n_fields, "integrator": "rk4", "h": h,
+                        "interp": "bilinear+linear-time", "field_dtype": "f32",
+                        "table": "tests/golden/batch64_table.json", "seeds_total": n_seeds,
+                        "seed_steps_total": seed_steps})
+
+
+WORKLOADS = {"abc128": build_abc128, "gyre512": build_gyre, "batch64": build_batch64}
 
 
 # ---------------------------------------------------------------------------
@@ -270,6 +345,74 @@ def load_traffic(name):
 
 # ---------------------------------------------------------------------------
 
+def run_batch(torch, wl, args):
+    """64 uneven double-gyre windows in one persistent, graph-captured launch."""
+    from pyftle_amd import window as W
+    from pyftle_amd.batch import BatchConfig, BatchPlan
+    from pyftle_amd.tuning import best_batch_config
+
+    cfg = best_batch_config("batch64") or BatchConfig()
+    plan = BatchPlan(wl["windows"], method="rk4", time_mode="linear", cfg=cfg)
+    info = plan.info()
+    log(f"batch: {info}")
+    for _ in range(args.warmup):
+        plan.run()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record(stream)
+        plan.run()
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [a.elapsed_time(b) for a, b in evs]
+    avg_ms = sum(kernel_ms) / len(kernel_ms)
+    log(f"launches (ms): {', '.join(f'{x:.2f}' for x in kernel_ms)}; wall {wall:.3f} s")
+    alg = wl["bytes_per_seed_step"] * wl["seed_steps"]
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    traffic = load_traffic("batch64")
+    result = {
+        "metric": METRIC, "value": wl["n_seeds"] * args.steps / wall, "unit": "seeds/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic", "config": wl["config"],
+        "seed_steps_per_s": wl["seed_steps"] * args.steps / wall,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic.get("hbm_bytes_per_launch") if traffic else None,
+                     "kernel": "batch_kernel<2,RK4,LINEAR,PAIR> (persistent, hipGraph replay)",
+                     "algorithmic_bytes_per_launch": alg, "bytes_per_seed_step": 1024,
+                     "avg_launch_ms": avg_ms},
+        "launch": dict(vars(cfg), **info),
+    }
+    if not args.no_accuracy:
+        log("batch parity against single-window launches and FP64")
+        same = True
+        worst = 0.0
+        checked = 0
+        for i, w in enumerate(wl["windows"]):
+            single = W.advect_ftle(w.field, w.positions, method="rk4", time_mode="linear",
+                                   n_steps=w.n_steps, h=w.h, map_period=w.map_period)
+            same = same and bool(torch.equal(single, w.ftle))
+            if checked < 6 and w.positions.shape[0] // 4 <= (1 << 17):
+                row = wl["table"][i]
+                f64 = _gyre_field_on_device(torch, row["grid"][0], row["grid"][1], w.field.n_snapshots,
+                                            row["eps"], row["omega"], dtype="float64")
+                ref = W.advect_ftle(f64, w.positions, method="rk4", time_mode="linear",
+                                    n_steps=w.n_steps, h=w.h, map_period=w.map_period)
+                rng_ = (ref.max() - ref.min()).item()
+                worst = max(worst, (w.ftle - ref).abs().max().item() / rng_)
+                checked += 1
+                del f64
+        result["accuracy"] = {"batch_equals_single_window_launches": same,
+                              "fp64_fields_checked": checked, "max_abs_err_over_range": worst}
+    result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+    plan.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -310,6 +453,8 @@ def main():
     torch.cuda.set_device(0)
     log(f"building workload {args.workload}")
     wl = WORKLOADS[args.workload](torch)
+    if wl["name"] == "batch64":
+        return run_batch(torch, wl, args)
     cfg = best_config(shape_class(wl["dim"], wl["n_seeds"], wl["time_mode"]))
     if args.groups_per_block > 0:
         cfg.groups_per_block = args.groups_per_block

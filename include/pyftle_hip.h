@@ -166,6 +166,49 @@ int pyftle_ftle_from_groups(const void *positions, const void *initial_positions
                             int32_t pos_dtype, int32_t dim, int64_t n, double map_period,
                             double *ftle_out, void *stream);
 
+/* ---- batched windows: many FTLE fields in one persistent launch ----
+ * No reference counterpart: pyFTLE runs one window per worker process
+ * (parallel.py:224-228, __main__.py:111-166); here every window of a batch is
+ * split into work items (field, block of seeds), ordered longest-first, and
+ * one persistent kernel steals items from a device counter.  All fields of a
+ * batch share dim, integrator, time mode, record layout and dtype; grids,
+ * snapshot rings, seed counts, windows and step sizes may all differ.
+ * pyftle_batch_create allocates the device work list and, with use_graph,
+ * captures counter reset + persistent kernel into a hipGraph; launches then
+ * replay it.  The fields' buffers must outlive the batch. */
+typedef struct pyftle_batch_field {
+  pyftle_field field;
+  const void *positions_in; /* block-ordered (2*dim*n_groups, dim) of pyftle_batch_opts.pos_dtype */
+  void *positions_out;      /* final positions (may be NULL) */
+  double *ftle_out;         /* (n_groups,) FP64 */
+  int64_t n_groups;
+  int32_t n_steps;
+  int32_t first_snapshot;
+  double t_start;
+  double h;
+  double map_period;
+} pyftle_batch_field;
+
+typedef struct pyftle_batch_opts {
+  int32_t method;           /* PYFTLE_EULER / PYFTLE_AB2 / PYFTLE_RK4 */
+  int32_t time_mode;        /* PYFTLE_TIME_* */
+  int32_t pos_dtype;        /* PYFTLE_F32 / PYFTLE_F64 */
+  int32_t groups_per_block; /* seeds per work item, 0 = 64 */
+  int32_t lane_order;       /* PYFTLE_ORDER_* */
+  int32_t blocks_per_cu;    /* persistent work-groups per CU, 0 = occupancy limit */
+  int32_t longest_first;    /* 1: items sorted by steps, longest first */
+  int32_t use_graph;        /* 1: replay a captured hipGraph */
+} pyftle_batch_opts;
+
+typedef struct pyftle_batch pyftle_batch; /* opaque */
+
+int pyftle_batch_create(const pyftle_batch_field *fields, int32_t n_fields,
+                        const pyftle_batch_opts *opts, pyftle_batch **out);
+int pyftle_batch_launch(pyftle_batch *batch, void *stream);
+int pyftle_batch_info(const pyftle_batch *batch, int64_t *n_items, int32_t *grid_blocks,
+                      int32_t *block_threads);
+int pyftle_batch_destroy(pyftle_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -423,9 +423,9 @@ __device__ __forceinline__ int wrap_cell(int &cell, int n, float inv_n) {
   return q;
 }
 
+// One work-group's share of a window: logical block lb of the seeds of `a`.
 template <int DIM, int METHOD, int TMODE, bool PAIR, bool FTLE>
-__global__ void __launch_bounds__(512) advect_kernel_cell(AdvectArgs a) {
-  const int64_t lb = logical_block(a.remap);
+__device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t lb) {
   const int t = threadIdx.x;
   const int nb = a.nb, G = a.G;
   int sl, j;
@@ -586,6 +586,11 @@ __global__ void __launch_bounds__(512) advect_kernel_cell(AdvectArgs a) {
     }
   }
   if constexpr (FTLE) ftle_epilogue<DIM>(a, lb, t, sl, j, fin, x0);
+}
+
+template <int DIM, int METHOD, int TMODE, bool PAIR, bool FTLE>
+__global__ void __launch_bounds__(512) advect_kernel_cell(AdvectArgs a) {
+  advect_block_cell<DIM, METHOD, TMODE, PAIR, FTLE>(a, logical_block(a.remap));
 }
 
 // ---------------------------------------------------------------------------

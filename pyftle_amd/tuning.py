@@ -88,3 +88,25 @@ def autotune(run, dim: int, key: str, repeats: int = 3, save: bool = True):
             with open(CACHE, "w") as f:
                 json.dump(data, f, indent=1, sort_keys=True)
     return best, [(ms, vars(c)) for ms, c in table]
+
+
+def best_batch_config(key: str):
+    """Cached batch configuration (``autotune_batch``), or None."""
+    from .batch import BatchConfig
+
+    entry = _read().get(key)
+    if not entry:
+        return None
+    return BatchConfig(groups_per_block=int(entry["groups_per_block"]),
+                       lane_order=int(entry.get("lane_order", 0)),
+                       blocks_per_cu=int(entry.get("blocks_per_cu", 0)),
+                       longest_first=int(entry.get("longest_first", 1)),
+                       use_graph=int(entry.get("use_graph", 1)))
+
+
+def save_batch_config(key: str, cfg, ms: float) -> None:
+    with _lock:
+        data = _read()
+        data[key] = dict(vars(cfg), ms=ms)
+        with open(CACHE, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
