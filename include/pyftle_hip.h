@@ -209,6 +209,50 @@ int pyftle_batch_info(const pyftle_batch *batch, int64_t *n_items, int32_t *grid
                       int32_t *block_threads);
 int pyftle_batch_destroy(pyftle_batch *batch);
 
+/* ---- flow-map surrogate: Fourier-feature MLP kernels ----
+ * No reference counterpart: a learned approximation of the flow map that
+ * pyFTLE's RK4 path (integrate.py:265-342) computes; RK4 stays the fallback
+ * and the judge.  BF16 operands, FP32 accumulation (MFMA 32x32x16). */
+#define PYFTLE_EPI_F32 0       /* C (FP32, split-K partials at C + split*M*ldc) = A B */
+#define PYFTLE_EPI_BIAS_SILU 1 /* Z = A B + bias (bf16, optional); C = SiLU(Z) (bf16) */
+#define PYFTLE_EPI_DSILU 2     /* C = (A B) * SiLU'(Z) (bf16), Z read at the same row modulo z_rows */
+#define PYFTLE_EPI_BIAS 3      /* C = A B (+ bias) (bf16) */
+
+/* C[M,N] = A[M,K] B[K,N].  a_k_contig: A stored row-major MxK (else A^T row-major);
+ * b_k_contig: B^T stored row-major NxK, e.g. a weight [out][in] (else B row-major KxN).
+ * M, N multiples of 128; K a multiple of 32*splits; leading dimensions multiples of 8.
+ * z_rows: row period of Z for EPI_DSILU (M for a plain backward pass; the primal
+ * row count when tangent blocks are stacked below the primal rows). */
+int pyftle_mlp_gemm(const void *A, int64_t lda, int32_t a_k_contig, const void *B, int64_t ldb,
+                    int32_t b_k_contig, int32_t M, int32_t N, int32_t K, int32_t splits,
+                    int32_t epilogue, const float *bias, void *Z, int32_t z_rows, void *C,
+                    int64_t ldc, void *stream);
+/* Fourier features [sin(2 pi B v), cos(2 pi B v)] of v = in*scale (B: F x d, FP32) as bf16
+ * rows of ld_out; with n_tangents, d/d in_a of the features for a < n_tangents in row
+ * blocks (1+a)*M .. of `out`. */
+int pyftle_mlp_fourier(const float *in, int32_t d, const float *scale, const float *Bm, int32_t F,
+                       int32_t M, int32_t ld_out, int32_t n_tangents, void *out, void *stream);
+/* Narrow output layer Y[M,O] = H W^T (+ b on the first rows_with_bias rows), FP32. */
+int pyftle_mlp_linear_out(const void *H, int64_t ldh, int32_t K, const float *W, const float *b,
+                          int32_t O, int32_t M, int32_t rows_with_bias, float *Y, void *stream);
+/* Its backward: dZ = (dY W) * SiLU'(Z) (bf16), and per-chunk partial dW (chunks x O x K)
+ * and db (chunks x O) over chunk_rows rows each. */
+int pyftle_mlp_linear_out_backward(const float *dY, const float *W, int32_t O, int32_t K, int32_t M,
+                                   const void *H, const void *Z, int64_t ld, int32_t chunk_rows,
+                                   float *dW_part, float *db_part, void *dZ, void *stream);
+int pyftle_mlp_reduce_splits(const float *part, int32_t splits, int64_t n, float *out, void *stream);
+/* Per-chunk partial column sums of a bf16 matrix (bias gradients): part (chunks x N). */
+int pyftle_mlp_colsum(const void *X, int64_t ld, int32_t M, int32_t N, int32_t chunk_rows, float *part,
+                      void *stream);
+/* dY = 2 (Y - T) * scale; loss += sum (Y - T)^2 * scale (FP32 atomics). */
+int pyftle_mlp_mse(const float *Y, const float *T, int64_t n, float scale, float *dY, float *loss,
+                   void *stream);
+/* Adam on FP32 master weights (bias-corrected, step >= 1), optional bf16 copy. */
+int pyftle_mlp_adam(float *w, float *m, float *v, const float *g, int64_t n, float lr, float beta1,
+                    float beta2, float eps, int32_t step, float weight_decay, void *w_bf16,
+                    void *stream);
+int pyftle_mlp_cast_bf16(const float *x, int64_t n, void *y, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,342 @@
+"""Learned flow-map surrogate: Fourier-feature MLP on the project's HIP kernels.
+
+The network maps ``(x, t0)`` (seed position and window start) to the
+displacement of the flow map over a fixed window ``T``:
+``Phi(x, t0) = x + NN(x, t0)``.  Its Jacobian ``dPhi/dx`` comes from
+forward-mode tangents (JVP) pushed through the same GEMMs, so the FTLE needs
+one primal and ``dim`` tangent row blocks per seed instead of ``2*dim``
+neighbour queries.  Training data are trajectories of the fused RK4 kernel;
+the FP64 FTLE kernels finish the job.  An ensemble of members flags seeds
+where the members disagree; those seeds fall back to RK4 (``FlowMapEnsemble``).
+
+Kernels (``csrc/mlp.hip``): BF16 MFMA GEMM with bias+SiLU / SiLU' epilogues
+and split-K weight gradients, Fourier features with analytic tangents, the
+narrow output layer, MSE and Adam.  Master weights are FP32; the GEMMs read a
+BF16 copy written by the Adam kernel.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .field import ptr, stream_handle
+
+EPI_F32, EPI_BIAS_SILU, EPI_DSILU, EPI_BIAS = 0, 1, 2, 3
+
+_vp, _i32, _i64, _f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_float
+_SIGS = {
+    "pyftle_mlp_gemm": [_vp, _i64, _i32, _vp, _i64, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _i32,
+                        _vp, _i64, _vp],
+    "pyftle_mlp_fourier": [_vp, _i32, _vp, _vp, _i32, _i32, _i32, _i32, _vp, _vp],
+    "pyftle_mlp_linear_out": [_vp, _i64, _i32, _vp, _vp, _i32, _i32, _i32, _vp, _vp],
+    "pyftle_mlp_linear_out_backward": [_vp, _vp, _i32, _i32, _i32, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp],
+    "pyftle_mlp_reduce_splits": [_vp, _i32, _i64, _vp, _vp],
+    "pyftle_mlp_colsum": [_vp, _i64, _i32, _i32, _i32, _vp, _vp],
+    "pyftle_mlp_mse": [_vp, _vp, _i64, _f32, _vp, _vp, _vp],
+    "pyftle_mlp_adam": [_vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _i32, _f32, _vp, _vp],
+    "pyftle_mlp_cast_bf16": [_vp, _i64, _vp, _vp],
+}
+_registered = False
+
+
+def _lib():
+    global _registered
+    if not _registered:
+        N.extra_signatures(_SIGS)
+        _registered = True
+    return N.load()
+
+
+def pad128(n: int) -> int:
+    return (n + 127) // 128 * 128
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def gemm(A, lda, a_kc, B, ldb, b_kc, M, Nn, K, C, ldc, epi=EPI_F32, splits=1, bias=None, Z=None,
+         z_rows=0, stream=None):
+    """``C = A B`` through ``pyftle_mlp_gemm`` (pointers may be tensor views)."""
+    _lib()
+    N.call("pyftle_mlp_gemm", A, int(lda), int(a_kc), B, int(ldb), int(b_kc), int(M), int(Nn), int(K),
+           int(splits), int(epi), bias, Z, int(z_rows), C, int(ldc), stream_handle(stream))
+
+
+def _off(t: torch.Tensor, elems: int) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr() + elems * t.element_size())
+
+
+@dataclass
+class MLPSpec:
+    d_in: int = 3
+    d_out: int = 2
+    n_freq: int = 64
+    width: int = 256
+    depth: int = 5
+    sigma: float = 1.0
+    scale: tuple = (0.5, 1.0, 0.1)
+    out_scale: float = 1.0
+
+    def n_params(self) -> int:
+        fan = 2 * self.n_freq
+        n = 0
+        for _ in range(self.depth):
+            n += self.width * fan + self.width
+            fan = self.width
+        return n + self.d_out * self.width + self.d_out
+
+
+class FlowMapMLP:
+    """One network: parameters in one flat FP32 buffer (+ BF16 copy)."""
+
+    def __init__(self, spec: MLPSpec, seed: int = 0, device: str = "cuda"):
+        _lib()
+        self.spec = spec
+        g = torch.Generator().manual_seed(seed)
+        dev = torch.device(device)
+        self.B = (torch.randn(spec.n_freq, spec.d_in, generator=g, dtype=torch.float64) * spec.sigma).float().to(dev)
+        self.scale = torch.tensor(spec.scale, dtype=torch.float32, device=dev)
+        self.n_params = spec.n_params()
+        self.params = torch.zeros(self.n_params, dtype=torch.float32, device=dev)
+        self.offsets = []  # (W offset, W shape, b offset) per hidden layer
+        o = 0
+        fan = 2 * spec.n_freq
+        for _ in range(spec.depth):
+            w = torch.randn(spec.width, fan, generator=g, dtype=torch.float64) * math.sqrt(2.0 / fan)
+            self.params[o:o + w.numel()] = w.reshape(-1).float().to(dev)
+            self.offsets.append((o, (spec.width, fan), o + w.numel()))
+            o += w.numel() + spec.width
+            fan = spec.width
+        wout = torch.randn(spec.d_out, spec.width, generator=g, dtype=torch.float64) * math.sqrt(1.0 / spec.width)
+        self.params[o:o + wout.numel()] = wout.reshape(-1).float().to(dev)
+        self.out_off = (o, (spec.d_out, spec.width), o + wout.numel())
+        self.params_bf16 = torch.empty(self.n_params, dtype=torch.bfloat16, device=dev)
+        self.sync_bf16()
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.grads = torch.zeros_like(self.params)
+        self.step = 0
+        self._ws = {}
+
+    # -- parameter views ---------------------------------------------------
+    def W(self, l):
+        o, shp, _ = self.offsets[l]
+        return self.params[o:o + shp[0] * shp[1]].view(shp)
+
+    def Wb(self, l):
+        o, shp, _ = self.offsets[l]
+        return self.params_bf16[o:o + shp[0] * shp[1]].view(shp)
+
+    def b(self, l):
+        _, shp, ob = self.offsets[l]
+        return self.params[ob:ob + shp[0]]
+
+    def Wout(self):
+        o, shp, _ = self.out_off
+        return self.params[o:o + shp[0] * shp[1]].view(shp)
+
+    def bout(self):
+        _, shp, ob = self.out_off
+        return self.params[ob:ob + shp[0]]
+
+    def sync_bf16(self):
+        N.call("pyftle_mlp_cast_bf16", _p(self.params), self.n_params, _p(self.params_bf16), stream_handle())
+
+    def to_numpy(self):
+        """Parameters as the oracle's dict (FP64 copies of the BF16 GEMM weights)."""
+        s = self.spec
+        return {
+            "B": self.B.double().cpu().numpy(), "scale": self.scale.double().cpu().numpy(),
+            "W": [self.Wb(l).double().cpu().numpy() for l in range(s.depth)],
+            "b": [self.b(l).double().cpu().numpy() for l in range(s.depth)],
+            "Wout": self.Wout().double().cpu().numpy() * s.out_scale,
+            "bout": self.bout().double().cpu().numpy() * s.out_scale,
+        }
+
+    def state_dict(self):
+        return {"spec": self.spec.__dict__, "B": self.B.cpu(), "params": self.params.cpu()}
+
+    def load_state(self, sd):
+        self.B.copy_(sd["B"].to(self.B.device))
+        self.params.copy_(sd["params"].to(self.params.device))
+        self.sync_bf16()
+
+    # -- workspace ---------------------------------------------------------
+    def _workspace(self, Mp: int, blocks: int):
+        key = (Mp, blocks)
+        ws = self._ws.get(key)
+        if ws is None:
+            s = self.spec
+            dev = self.params.device
+            R = Mp * blocks
+            ws = {
+                "x": torch.zeros(Mp, s.d_in, dtype=torch.float32, device=dev),
+                "F": torch.empty(R, 2 * s.n_freq, dtype=torch.bfloat16, device=dev),
+                "H": [torch.empty(R, s.width, dtype=torch.bfloat16, device=dev) for _ in range(s.depth)],
+                "Z": [torch.empty(Mp, s.width, dtype=torch.bfloat16, device=dev) for _ in range(s.depth)],
+                "Y": torch.empty(R, s.d_out, dtype=torch.float32, device=dev),
+            }
+            self._ws = {key: ws}  # keep one workspace size resident
+        return ws
+
+    # -- forward / JVP -----------------------------------------------------
+    def forward(self, x: torch.Tensor, n_tangents: int = 0):
+        """Outputs ``(M, d_out)`` and, with ``n_tangents``, d out / d x_a for a < n_tangents."""
+        s = self.spec
+        M = x.shape[0]
+        Mp = pad128(max(M, 1))
+        blocks = 1 + n_tangents
+        ws = self._workspace(Mp, blocks)
+        ws["x"][:M].copy_(x)
+        if M < Mp:
+            ws["x"][M:].zero_()
+        F2 = 2 * s.n_freq
+        N.call("pyftle_mlp_fourier", _p(ws["x"]), s.d_in, _p(self.scale), _p(self.B), s.n_freq, Mp, F2,
+               n_tangents, _p(ws["F"]), stream_handle())
+        h, fan = ws["F"], F2
+        for l in range(s.depth):
+            H = ws["H"][l]
+            gemm(_p(h), fan, 1, _p(self.Wb(l)), fan, 1, Mp, s.width, fan, _p(H), s.width,
+                 epi=EPI_BIAS_SILU, bias=_p(self.b(l)), Z=_p(ws["Z"][l]))
+            if n_tangents:
+                gemm(_off(h, Mp * fan), fan, 1, _p(self.Wb(l)), fan, 1, Mp * n_tangents, s.width, fan,
+                     _off(H, Mp * s.width), s.width, epi=EPI_DSILU, Z=_p(ws["Z"][l]), z_rows=Mp)
+            h, fan = H, s.width
+        N.call("pyftle_mlp_linear_out", _p(h), s.width, s.width, _p(self.Wout()), _p(self.bout()), s.d_out,
+               Mp * blocks, Mp, _p(ws["Y"]), stream_handle())
+        Y = ws["Y"].view(blocks, Mp, s.d_out)[:, :M] * s.out_scale
+        if n_tangents:
+            return Y[0], [Y[1 + a] for a in range(n_tangents)]
+        return Y[0]
+
+    # -- training ----------------------------------------------------------
+    def loss_and_backward(self, x: torch.Tensor, target: torch.Tensor, loss_out: torch.Tensor | None = None):
+        """MSE over all outputs; fills ``self.grads``; returns the loss tensor (device)."""
+        s = self.spec
+        M = x.shape[0]
+        Mp = pad128(M)
+        self.forward(x)
+        ws = self._ws[(Mp, 1)]
+        dY = torch.zeros(Mp, s.d_out, dtype=torch.float32, device=x.device)
+        loss = loss_out if loss_out is not None else torch.zeros(1, dtype=torch.float32, device=x.device)
+        # the network output before out_scale is Y/out_scale; fit target/out_scale
+        tgt = (target / s.out_scale).contiguous()
+        N.call("pyftle_mlp_mse", _p(ws["Y"]), _p(tgt), M * s.d_out, 1.0 / (M * s.d_out), _p(dY), _p(loss),
+               stream_handle())
+        self.backward(dY, Mp)
+        return loss
+
+    def backward(self, dY: torch.Tensor, Mp: int):
+        s = self.spec
+        ws = self._ws[(Mp, 1)]
+        dev = dY.device
+        chunk = 4096
+        chunks = (Mp + chunk - 1) // chunk
+        g = self.grads
+        o, shp, ob = self.out_off
+        dWp = torch.empty(chunks, s.d_out, s.width, dtype=torch.float32, device=dev)
+        dbp = torch.empty(chunks, s.d_out, dtype=torch.float32, device=dev)
+        dZ = torch.empty(Mp, s.width, dtype=torch.bfloat16, device=dev)
+        hlast = ws["H"][s.depth - 1]
+        N.call("pyftle_mlp_linear_out_backward", _p(dY), _p(self.Wout()), s.d_out, s.width, Mp, _p(hlast),
+               _p(ws["Z"][s.depth - 1]), s.width, chunk, _p(dWp), _p(dbp), _p(dZ), stream_handle())
+        N.call("pyftle_mlp_reduce_splits", _p(dWp), chunks, s.d_out * s.width, _off(g, o), stream_handle())
+        N.call("pyftle_mlp_reduce_splits", _p(dbp), chunks, s.d_out, _off(g, ob), stream_handle())
+        splits = max(1, Mp // chunk)
+        while Mp % (splits * 32):
+            splits -= 1
+        F2 = 2 * s.n_freq
+        for l in range(s.depth - 1, -1, -1):
+            o, (wo, wi), ob = self.offsets[l]
+            hin = ws["F"] if l == 0 else ws["H"][l - 1]
+            part = torch.empty(splits, wo, wi, dtype=torch.float32, device=dev)
+            gemm(_p(dZ), s.width, 0, _p(hin), wi, 0, wo, wi, Mp, _p(part), wi, epi=EPI_F32, splits=splits)
+            N.call("pyftle_mlp_reduce_splits", _p(part), splits, wo * wi, _off(g, o), stream_handle())
+            cp = torch.empty(chunks, wo, dtype=torch.float32, device=dev)
+            N.call("pyftle_mlp_colsum", _p(dZ), s.width, Mp, wo, chunk, _p(cp), stream_handle())
+            N.call("pyftle_mlp_reduce_splits", _p(cp), chunks, wo, _off(g, ob), stream_handle())
+            if l > 0:
+                dZp = torch.empty(Mp, wi, dtype=torch.bfloat16, device=dev)
+                gemm(_p(dZ), s.width, 1, _p(self.Wb(l)), wi, 0, Mp, wi, wo, _p(dZp), wi,
+                     epi=EPI_DSILU, Z=_p(ws["Z"][l - 1]), z_rows=Mp)
+                dZ = dZp
+        return g
+
+    def adam_step(self, lr: float, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0):
+        self.step += 1
+        N.call("pyftle_mlp_adam", _p(self.params), _p(self.m), _p(self.v), _p(self.grads), self.n_params,
+               float(lr), float(beta1), float(beta2), float(eps), self.step, float(weight_decay),
+               _p(self.params_bf16), stream_handle())
+
+
+def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20, batch: int = 32768,
+          lr: float = 2e-3, lr_min: float = 2e-5, seed: int = 0, log=None):
+    """Adam with a cosine learning-rate schedule over minibatches; returns per-epoch losses."""
+    n = X.shape[0]
+    steps_per_epoch = max(1, n // batch)
+    total = epochs * steps_per_epoch
+    gen = torch.Generator(device=X.device).manual_seed(seed)
+    losses = []
+    k = 0
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=gen, device=X.device)
+        acc = torch.zeros(1, dtype=torch.float32, device=X.device)
+        for i in range(steps_per_epoch):
+            idx = perm[i * batch:(i + 1) * batch]
+            model.loss_and_backward(X[idx].contiguous(), T[idx].contiguous(), loss_out=acc)
+            cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * k / max(1, total)))
+            model.adam_step(cur)
+            k += 1
+        losses.append(acc.item() / steps_per_epoch)
+        if log:
+            log(f"epoch {ep} loss {losses[-1]:.3e}")
+    return losses
+
+
+class FlowMapEnsemble:
+    """Members trained on the same trajectories from different seeds."""
+
+    def __init__(self, members: list[FlowMapMLP]):
+        self.members = members
+
+    def predict(self, x: torch.Tensor, dim: int):
+        """Mean displacement, mean Jacobian of the displacement and member spread."""
+        outs, jacs = [], []
+        for m in self.members:
+            y, tans = m.forward(x, n_tangents=dim)
+            outs.append(y.clone())
+            jacs.append(torch.stack([t.clone() for t in tans], dim=2))  # (M, d_out, dim): d y_r / d x_c
+        Y = torch.stack(outs)
+        Jd = torch.stack(jacs)
+        mean = Y.mean(0)
+        spread = (Y - mean).abs().amax(dim=(0, 2))
+        return mean, Jd.mean(0), spread
+
+
+def ftle_from_displacement_jacobian(Jd: torch.Tensor, T: float) -> torch.Tensor:
+    """FTLE of the flow map x + d(x): J = I + dd/dx, FP64 closed-form kernels."""
+    dim = Jd.shape[1]
+    J = (Jd.double() + torch.eye(dim, dtype=torch.float64, device=Jd.device)).contiguous()
+    out = torch.empty(J.shape[0], dtype=torch.float64, device=J.device)
+    N.call("pyftle_ftle_from_jacobian", _p(J), dim, J.shape[0], float(T), _p(out), stream_handle())
+    return out
+
+
+def mlp_flops_per_query(spec: MLPSpec) -> int:
+    fan = 2 * spec.n_freq
+    f = 0
+    for _ in range(spec.depth):
+        f += 2 * fan * spec.width
+        fan = spec.width
+    return f + 2 * spec.width * spec.d_out
+
+
+__all__ = ["MLPSpec", "FlowMapMLP", "FlowMapEnsemble", "train", "ftle_from_displacement_jacobian",
+           "gemm", "pad128", "mlp_flops_per_query", "np"]

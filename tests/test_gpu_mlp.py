@@ -1,0 +1,155 @@
+"""GPU: surrogate MLP kernels against torch FP32 products and the FP64 MLP oracle.
+
+Tolerances: GEMMs take BF16 operands and accumulate in FP32, so they are
+compared with an FP32 product of the same BF16-rounded operands (rtol 1e-3,
+summation order only).  Whole-network values, gradients and JVPs round every
+activation to BF16 and are compared norm-wise against the FP64 oracle
+(relative error <= 3e-2).
+"""
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as MO
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _lib():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from pyftle_amd import surrogate
+
+    surrogate._lib()
+
+
+def _bf(t):
+    return t.to(torch.bfloat16).contiguous()
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+@pytest.mark.parametrize("a_kc,b_kc", [(1, 1), (1, 0), (0, 1), (0, 0)])
+def test_gemm_layouts(a_kc, b_kc):
+    from pyftle_amd.surrogate import EPI_F32, _p, gemm
+
+    g = torch.Generator(device="cuda").manual_seed(1)
+    M, N, K = 256, 384, 160
+    A = torch.randn(M, K, device="cuda", generator=g)
+    B = torch.randn(K, N, device="cuda", generator=g)
+    Ab = _bf(A if a_kc else A.t())
+    Bb = _bf(B.t() if b_kc else B)
+    C = torch.empty(M, N, device="cuda")
+    gemm(_p(Ab), Ab.shape[1], a_kc, _p(Bb), Bb.shape[1], b_kc, M, N, K, _p(C), N, epi=EPI_F32)
+    Ar = Ab.float() if a_kc else Ab.float().t()
+    Br = Bb.float().t() if b_kc else Bb.float()
+    ref = Ar @ Br
+    torch.testing.assert_close(C, ref, rtol=1e-3, atol=1e-3)
+
+
+def test_gemm_split_k_and_epilogues():
+    from pyftle_amd.surrogate import EPI_BIAS_SILU, EPI_DSILU, EPI_F32, _p, gemm
+
+    g = torch.Generator(device="cuda").manual_seed(2)
+    M, N, K = 256, 128, 512
+    A = _bf(torch.randn(M, K, device="cuda", generator=g))
+    W = _bf(torch.randn(N, K, device="cuda", generator=g) * 0.05)
+    bias = torch.randn(N, device="cuda", generator=g)
+    ref = A.float() @ W.float().t()
+    part = torch.empty(4, M, N, device="cuda")
+    gemm(_p(A), K, 1, _p(W), K, 1, M, N, K, _p(part), N, epi=EPI_F32, splits=4)
+    torch.testing.assert_close(part.sum(0), ref, rtol=1e-3, atol=1e-3)
+    H = torch.empty(M, N, dtype=torch.bfloat16, device="cuda")
+    Z = torch.empty(M, N, dtype=torch.bfloat16, device="cuda")
+    gemm(_p(A), K, 1, _p(W), K, 1, M, N, K, _p(H), N, epi=EPI_BIAS_SILU, bias=_p(bias), Z=_p(Z))
+    z = ref + bias
+    torch.testing.assert_close(Z.float(), z, rtol=1e-2, atol=1e-2)
+    torch.testing.assert_close(H.float(), torch.nn.functional.silu(z), rtol=1e-2, atol=1e-2)
+    # SiLU' epilogue with the row period of stacked tangent blocks (2 blocks of 128 rows)
+    D = torch.empty(M, N, dtype=torch.bfloat16, device="cuda")
+    gemm(_p(A), K, 1, _p(W), K, 1, M, N, K, _p(D), N, epi=EPI_DSILU, Z=_p(Z), z_rows=128)
+    zz = Z.float()[:128].repeat(2, 1)
+    s = torch.sigmoid(zz)
+    torch.testing.assert_close(D.float(), ref * s * (1 + zz * (1 - s)), rtol=2e-2, atol=2e-2)
+
+
+def test_gemm_rejects_bad_shapes():
+    from pyftle_amd import _native
+    from pyftle_amd.surrogate import _p, gemm
+
+    A = torch.zeros(100, 64, dtype=torch.bfloat16, device="cuda")
+    with pytest.raises(_native.PyftleError, match="multiples of 128"):
+        gemm(_p(A), 64, 1, _p(A), 64, 1, 100, 128, 64, _p(A), 128)
+
+
+def _small_model(seed=0, width=128, depth=3):
+    from pyftle_amd.surrogate import FlowMapMLP, MLPSpec
+
+    spec = MLPSpec(d_in=3, d_out=2, n_freq=64, width=width, depth=depth, sigma=1.5, scale=(0.5, 1.0, 0.1))
+    return FlowMapMLP(spec, seed=seed)
+
+
+def test_forward_and_jvp_match_oracle():
+    m = _small_model()
+    p = m.to_numpy()
+    rng = np.random.default_rng(0)
+    x = rng.uniform([0, 0, 0], [2, 1, 10], (300, 3))
+    xt = torch.as_tensor(x, dtype=torch.float32, device="cuda")
+    y, tans = m.forward(xt, n_tangents=2)
+    assert _rel(y.cpu().numpy(), MO.forward(p, x)) < 3e-2
+    for a in range(2):
+        assert _rel(tans[a].cpu().numpy(), MO.jvp(p, x, a)) < 3e-2
+
+
+def test_gradients_match_oracle():
+    m = _small_model(seed=1)
+    p = m.to_numpy()
+    rng = np.random.default_rng(1)
+    x = rng.uniform([0, 0, 0], [2, 1, 10], (500, 3))
+    t = rng.standard_normal((500, 2)) * 0.3
+    loss = m.loss_and_backward(torch.as_tensor(x, dtype=torch.float32, device="cuda"),
+                               torch.as_tensor(t, dtype=torch.float32, device="cuda"))
+    l_ref, g_ref = MO.loss_and_grads(p, x, t)
+    assert abs(loss.item() - l_ref) / l_ref < 3e-2
+    g = m.grads
+    for l in range(m.spec.depth):
+        o, shp, ob = m.offsets[l]
+        gw = g[o:o + shp[0] * shp[1]].view(shp).cpu().numpy()
+        gb = g[ob:ob + shp[0]].cpu().numpy()
+        assert _rel(gw, g_ref["W"][l]) < 3e-2, l
+        assert _rel(gb, g_ref["b"][l]) < 3e-2, l
+    o, shp, ob = m.out_off
+    assert _rel(g[o:o + shp[0] * shp[1]].view(shp).cpu().numpy(), g_ref["Wout"]) < 3e-2
+    assert _rel(g[ob:ob + 2].cpu().numpy(), g_ref["bout"]) < 3e-2
+
+
+def test_adam_matches_formula():
+    m = _small_model(seed=2)
+    w0 = m.params.clone()
+    m.grads.copy_(torch.randn_like(m.grads))
+    gr = m.grads.clone()
+    m.adam_step(1e-2)
+    mm = 0.1 * gr
+    vv = 0.001 * gr * gr
+    exp = w0 - 1e-2 * (mm / 0.1) / (torch.sqrt(vv / 0.001) + 1e-8)
+    torch.testing.assert_close(m.params, exp, rtol=1e-5, atol=1e-6)
+    torch.testing.assert_close(m.params_bf16.float(), m.params, rtol=1e-2, atol=1e-2)
+
+
+def test_training_reduces_loss():
+    from pyftle_amd.surrogate import train
+
+    m = _small_model(seed=3)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    X = torch.rand(8192, 3, device="cuda", generator=g) * torch.tensor([2.0, 1.0, 10.0], device="cuda")
+    T = torch.stack([torch.sin(np.pi * X[:, 0]) * 0.3, torch.cos(np.pi * X[:, 1]) * 0.2 * torch.sin(X[:, 2])], 1)
+    losses = train(m, X, T, epochs=15, batch=2048, lr=3e-3)
+    print(f"[gpu] loss {losses[0]:.3e} -> {losses[-1]:.3e}", flush=True)
+    assert losses[-1] < 0.1 * losses[0]
