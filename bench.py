@@ -200,7 +200,94 @@ def build_batch64(torch, h=0.05, n_fields=64):
                         "seed_steps_total": seed_steps})
 
 
-WORKLOADS = {"abc128": build_abc128, "gyre512": build_gyre, "batch64": build_batch64}
+def build_surrogate2d(torch, nsx=2048, nsy=1024, n_t0=32, members=4, epochs=80, n_traj=4_000_000):
+    """Double-gyre flow-map surrogate: 512x256 snapshots over t in [0, 20] (the flow is
+    10-periodic, windows start in [0, 10)), RK4 h=0.05 over T=10."""
+    from pyftle_amd import surrogate_ftle as SF
+
+    field = _gyre_field_on_device(torch, 512, 256, 201, 0.25, 2.0 * math.pi / 10.0)
+    return dict(name="surrogate2d", field=field, nsx=nsx, nsy=nsy, n_t0=n_t0, members=members,
+                epochs=epochs, n_traj=n_traj, T=10.0, h=0.05, dim=2, SF=SF,
+                config={"workload": "surrogate2d", "flow": "double gyre A=0.1 eps=0.25 omega=2pi/10",
+                        "grid": [512, 256], "snapshots": 201, "seeds": [nsx, nsy], "t0_values": n_t0,
+                        "T": 10.0, "rk4_h": 0.05, "mlp": "64 Fourier freqs on (x,y,t0), 5x256 SiLU -> 2",
+                        "members": members, "epochs": epochs, "trajectories": n_traj,
+                        "fallback_tol": 1e-3, "dtype": "bf16 MFMA, fp32 accumulate"})
+
+
+def run_surrogate(torch, wl, args):
+    """Surrogate FTLE fields (ensemble + RK4 fallback) against plain GPU RK4 on the
+    same seed-windows; both measured with HIP events on the current stream."""
+    from pyftle_amd import surrogate as S
+
+    SF = wl["SF"]
+    field = wl["field"]
+    t = time.perf_counter()
+    X, D = SF.make_training_data(field, wl["n_traj"], wl["T"], wl["h"], (0.0, 10.0), (0.0, 0.0), (2.0, 1.0),
+                                 n_bins=500)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t
+    spec = S.MLPSpec(sigma=1.0)
+    t = time.perf_counter()
+    ens = SF.fit_ensemble(X, D, spec, members=wl["members"], epochs=wl["epochs"], batch=16384, lr=3e-3, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t
+    del X, D
+    nsx, nsy = wl["nsx"], wl["nsy"]
+    dev = torch.device("cuda")
+    xs = (torch.arange(nsx, device=dev, dtype=torch.float64) + 0.5) * (2.0 / nsx)
+    ys = (torch.arange(nsy, device=dev, dtype=torch.float64) + 0.5) * (1.0 / nsy)
+    cy, cx = torch.meshgrid(ys, xs, indexing="ij")
+    seeds = torch.stack([cx.reshape(-1), cy.reshape(-1)], 1).contiguous()
+    t0s = [10.0 * k / wl["n_t0"] for k in range(wl["n_t0"])]
+    sf = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"])
+    n_sw = seeds.shape[0] * len(t0s)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            out = fn()
+        b.record()
+        torch.cuda.synchronize()
+        return out, a.elapsed_time(b) / reps
+
+    plan, wins = SF.rk4_ftle_windows(field, seeds, t0s, wl["T"], wl["h"])
+    _, rk4_ms = timed(lambda: plan.run(), max(1, args.steps))
+    results, sur_ms = timed(lambda: [sf.ftle(seeds, t0) for t0 in t0s], max(1, args.steps))
+    preds, pred_ms = timed(lambda: [sf.predict(seeds, t0) for t0 in t0s], 1)
+    rmse = [SF.relative_rmse(r.ftle, w.ftle) for r, w in zip(results, wins)]
+    rmse_raw = [SF.relative_rmse(p[0], w.ftle) for p, w in zip(preds, wins)]
+    fb = sum(int(r.fallback.sum().item()) for r in results) / n_sw
+    rows = n_sw * wl["members"] * 3
+    flops = rows * S.mlp_flops_per_query(spec)
+    result = {
+        "metric": METRIC, "value": n_sw / (sur_ms * 1e-3), "unit": "seeds/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": sur_ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": wl["config"],
+        "surrogate": {"ms_per_field_set": sur_ms, "seed_windows": n_sw,
+                      "ftle_rmse_over_range_mean": sum(rmse) / len(rmse), "ftle_rmse_over_range_max": max(rmse),
+                      "ftle_rmse_over_range_without_fallback_mean": sum(rmse_raw) / len(rmse_raw),
+                      "fallback_fraction": fb, "surrogate_only_ms": pred_ms,
+                      "mfma": {"achieved": flops / (pred_ms * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
+                               "frac": flops / (pred_ms * 1e-3) / 1e12 / 2500.0,
+                               "flops_per_query": S.mlp_flops_per_query(spec), "queries": rows},
+                      "train_s": t_train, "data_s": t_data},
+        "rk4": {"ms_per_field_set": rk4_ms, "seeds_per_s": n_sw / (rk4_ms * 1e-3),
+                "seed_steps_per_s": n_sw * round(wl["T"] / wl["h"]) / (rk4_ms * 1e-3)},
+        "speedup_vs_gpu_rk4": rk4_ms / sur_ms,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+    plan.close()
+
+
+WORKLOADS = {"abc128": build_abc128, "gyre512": build_gyre, "batch64": build_batch64,
+             "surrogate2d": build_surrogate2d}
 
 
 # ---------------------------------------------------------------------------
@@ -455,6 +542,8 @@ def main():
     wl = WORKLOADS[args.workload](torch)
     if wl["name"] == "batch64":
         return run_batch(torch, wl, args)
+    if wl["name"] == "surrogate2d":
+        return run_surrogate(torch, wl, args)
     cfg = best_config(shape_class(wl["dim"], wl["n_seeds"], wl["time_mode"]))
     if args.groups_per_block > 0:
         cfg.groups_per_block = args.groups_per_block

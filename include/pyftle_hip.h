@@ -220,7 +220,7 @@ int pyftle_batch_destroy(pyftle_batch *batch);
 
 /* C[M,N] = A[M,K] B[K,N].  a_k_contig: A stored row-major MxK (else A^T row-major);
  * b_k_contig: B^T stored row-major NxK, e.g. a weight [out][in] (else B row-major KxN).
- * M, N multiples of 128; K a multiple of 32*splits; leading dimensions multiples of 8.
+ * M, N multiples of 128; K a multiple of 64*splits; leading dimensions multiples of 8.
  * z_rows: row period of Z for EPI_DSILU (M for a plain backward pass; the primal
  * row count when tangent blocks are stacked below the primal rows). */
 int pyftle_mlp_gemm(const void *A, int64_t lda, int32_t a_k_contig, const void *B, int64_t ldb,

@@ -46,34 +46,50 @@ __device__ __forceinline__ float dsilu_f(float z) {
   return s * (1.0f + z * (1.0f - s));
 }
 
-constexpr int BM = 128, BN = 128, BK = 32, LDSK = BK + 8;  // padded LDS rows (bank spread)
+constexpr int BM = 128, BN = 128, BK = 64, LDSK = BK + 8;  // padded LDS rows (bank spread)
 
-// Stage a 128 x 32 (rows x k) tile into LDS rows-major with k contiguous.
-// KC: the source is k-contiguous (element (r,k) at src[r*ld + k]); otherwise
-// r-contiguous (element (r,k) at src[k*ld + r]).
+// A 128 x 64 (rows x k) operand tile held in registers between its global
+// load and its LDS store, so the next tile's loads overlap the MFMAs of the
+// current one.  KC: the source is k-contiguous (element (r,k) at
+// src[r*ld + k]); otherwise r-contiguous (element (r,k) at src[k*ld + r]).
 template <bool KC>
-__device__ __forceinline__ void stage_tile(__bf16 (*sm)[LDSK], const __bf16 *src, int64_t ld, int r0,
-                                           int k0, int tid) {
+struct TileLoader {
+  bf16x8 v[4];
+  __device__ __forceinline__ void load(const __bf16 *src, int64_t ld, int r0, int k0, int tid) {
 #pragma unroll
-  for (int rep = 0; rep < 2; ++rep) {
-    const int c = tid + 256 * rep;  // 512 chunks of 8 elements
-    if (KC) {
-      const int row = c >> 2, kq = (c & 3) * 8;
-      const bf16x8 v = *reinterpret_cast<const bf16x8 *>(src + (int64_t)(r0 + row) * ld + k0 + kq);
-      *reinterpret_cast<bf16x8 *>(&sm[row][kq]) = v;
-    } else {
-      const int k = c >> 4, rq = (c & 15) * 8;
-      const bf16x8 v = *reinterpret_cast<const bf16x8 *>(src + (int64_t)(k0 + k) * ld + r0 + rq);
-#pragma unroll
-      for (int e = 0; e < 8; ++e) sm[rq + e][k] = v[e];
+    for (int rep = 0; rep < 4; ++rep) {
+      const int c = tid + 256 * rep;  // 1024 chunks of 8 elements
+      if (KC) {
+        const int row = c >> 3, kq = (c & 7) * 8;
+        v[rep] = *reinterpret_cast<const bf16x8 *>(src + (int64_t)(r0 + row) * ld + k0 + kq);
+      } else {
+        const int k = c >> 4, rq = (c & 15) * 8;
+        v[rep] = *reinterpret_cast<const bf16x8 *>(src + (int64_t)(k0 + k) * ld + r0 + rq);
+      }
     }
   }
-}
+  __device__ __forceinline__ void store(__bf16 (*sm)[LDSK], int tid) const {
+#pragma unroll
+    for (int rep = 0; rep < 4; ++rep) {
+      const int c = tid + 256 * rep;
+      if (KC) {
+        const int row = c >> 3, kq = (c & 7) * 8;
+        *reinterpret_cast<bf16x8 *>(&sm[row][kq]) = v[rep];
+      } else {
+        const int k = c >> 4, rq = (c & 15) * 8;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) sm[rq + e][k] = v[rep][e];
+      }
+    }
+  }
+};
 
 template <bool A_KC, bool B_KC, int EPI>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
-  __shared__ __bf16 As[BM][LDSK];
-  __shared__ __bf16 Bs[BN][LDSK];
+  // One LDS buffer: operand tiles in the main loop, the output tile after it.
+  __shared__ __attribute__((aligned(16))) __bf16 smem[(BM + BN) * LDSK];
+  __bf16(*As)[LDSK] = reinterpret_cast<__bf16(*)[LDSK]>(smem);
+  __bf16(*Bs)[LDSK] = reinterpret_cast<__bf16(*)[LDSK]>(smem + BM * LDSK);
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -89,10 +105,19 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
       for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
 
   const int lr = lane & 31, lk = (lane >> 5) * 8;
+  TileLoader<A_KC> la;
+  TileLoader<B_KC> lb;
+  la.load(g.A, g.lda, m0, kb, tid);
+  lb.load(g.B, g.ldb, n0, kb, tid);
+  la.store(As, tid);
+  lb.store(Bs, tid);
+  __syncthreads();
   for (int k0 = kb; k0 < ke; k0 += BK) {
-    stage_tile<A_KC>(As, g.A, g.lda, m0, k0, tid);
-    stage_tile<B_KC>(Bs, g.B, g.ldb, n0, k0, tid);
-    __syncthreads();
+    const bool more = k0 + BK < ke;
+    if (more) {
+      la.load(g.A, g.lda, m0, k0 + BK, tid);
+      lb.load(g.B, g.ldb, n0, k0 + BK, tid);
+    }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 16) {
       bf16x8 a[2], b[2];
@@ -106,31 +131,64 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
         for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
+    if (more) {
+      la.store(As, tid);
+      lb.store(Bs, tid);
+      __syncthreads();
+    }
   }
 
+  if constexpr (EPI == EPI_F32) {
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int row = m0 + wm * 64 + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
-        const int col = n0 + wn * 64 + j * 32 + (lane & 31);
-        const float c = acc[i][j][e];
-        const int64_t o = (int64_t)row * g.ldc + col;
-        if (EPI == EPI_F32) {
-          g.C32[(int64_t)blockIdx.z * g.M * g.ldc + o] = c;
-        } else if (EPI == EPI_BIAS_SILU) {
-          const float z = c + g.bias[col];
-          if (g.Z) g.Z[o] = (__bf16)z;
-          g.Cb[o] = (__bf16)silu_f(z);
-        } else if (EPI == EPI_DSILU) {
-          const int zr = row - (m0 / g.z_rows) * g.z_rows;  // tiles never straddle a block
-          g.Cb[o] = (__bf16)(c * dsilu_f((float)g.Z[(int64_t)zr * g.ldc + col]));
-        } else {
-          g.Cb[o] = (__bf16)(c + (g.bias ? g.bias[col] : 0.0f));
+        for (int e = 0; e < 16; ++e) {
+          const int row = m0 + wm * 64 + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+          const int col = n0 + wn * 64 + j * 32 + (lane & 31);
+          g.C32[(int64_t)blockIdx.z * g.M * g.ldc + (int64_t)row * g.ldc + col] = acc[i][j][e];
         }
+  } else {
+    // Stage the bf16 tile in LDS (the operand buffers are free after the
+    // main loop), then write 16-byte chunks per lane, coalesced along rows.
+    static_assert(BM * (BN + 8) <= (BM + BN) * LDSK, "the output tile fits in the operand buffer");
+    __bf16(*Ct)[BN + 8] = reinterpret_cast<__bf16(*)[BN + 8]>(smem);  // EPI_BIAS_SILU: Z, then SiLU(Z)
+#pragma unroll
+    for (int pass = 0; pass < (EPI == EPI_BIAS_SILU ? 2 : 1); ++pass) {
+      if (EPI == EPI_BIAS_SILU && pass == 0 && !g.Z) continue;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+            const int r = wm * 64 + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
+            const int cc = wn * 64 + j * 32 + (lane & 31);
+            float v = acc[i][j][e];
+            if (EPI == EPI_BIAS_SILU || EPI == EPI_BIAS) v += (g.bias ? g.bias[n0 + cc] : 0.0f);
+            if (EPI == EPI_BIAS_SILU && pass == 1) v = silu_f(v);
+            Ct[r][cc] = (__bf16)v;
+          }
+      __syncthreads();
+      __bf16 *dst = (EPI == EPI_BIAS_SILU && pass == 0) ? g.Z : g.Cb;
+      const int zoff = (EPI == EPI_DSILU) ? (m0 / g.z_rows) * g.z_rows : 0;
+#pragma unroll
+      for (int rep = 0; rep < 8; ++rep) {
+        const int c = tid + 256 * rep;  // 2048 chunks of 8
+        const int r = c >> 4, cq = (c & 15) * 8;
+        bf16x8 v = *reinterpret_cast<const bf16x8 *>(&Ct[r][cq]);
+        const int64_t o = (int64_t)(m0 + r) * g.ldc + n0 + cq;
+        if (EPI == EPI_DSILU) {
+          const bf16x8 z = *reinterpret_cast<const bf16x8 *>(g.Z + (int64_t)(m0 + r - zoff) * g.ldc + n0 + cq);
+#pragma unroll
+          for (int q = 0; q < 8; ++q) v[q] = (__bf16)((float)v[q] * dsilu_f((float)z[q]));
+        }
+        *reinterpret_cast<bf16x8 *>(dst + o) = v;
       }
+      __syncthreads();
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -146,13 +204,14 @@ __global__ void __launch_bounds__(256) fourier_kernel(const float *in, int d, co
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)M * F) return;
   const int m = (int)(idx / F), f = (int)(idx % F);
-  float ph = 0.0f;
+  float ph = 0.0f;  // phase in revolutions
   for (int a = 0; a < d; ++a) ph += Bm[f * d + a] * in[(int64_t)m * d + a] * scale[a];
-  const float tp = 6.283185307179586f;
-  float s, c;
-  sincosf(tp * ph, &s, &c);
+  ph -= rintf(ph);  // the hardware sine takes revolutions in a bounded range
+  const float s = __builtin_amdgcn_sinf(ph);  // sin(2 pi ph)
+  const float c = __builtin_amdgcn_cosf(ph);
   out[(int64_t)m * ld_out + f] = (__bf16)s;
   out[(int64_t)m * ld_out + F + f] = (__bf16)c;
+  const float tp = 6.283185307179586f;
   for (int a = 0; a < n_tan; ++a) {
     const float dph = tp * Bm[f * d + a] * scale[a];
     const int64_t r = (int64_t)(a + 1) * M + m;
@@ -162,20 +221,31 @@ __global__ void __launch_bounds__(256) fourier_kernel(const float *in, int d, co
 }
 
 // Narrow output layer y[m, o] = sum_k h[m, k] W[o, k] (+ b[o]), FP32 out.
-// One wave per row; `rows_with_bias` rows get the bias (tangent rows do not).
+// One thread per row, 16-byte loads of the row, W staged in LDS (O*K <= 4096);
+// `rows_with_bias` rows get the bias (tangent rows do not).
 __global__ void __launch_bounds__(256) linear_out_kernel(const __bf16 *H, int64_t ldh, int K,
                                                          const float *W, const float *b, int O, int M,
                                                          int rows_with_bias, float *Y) {
-  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (wave >= M) return;
-  for (int o = 0; o < O; ++o) {
-    float s = 0.0f;
-    for (int k = lane; k < K; k += 64) s += (float)H[(int64_t)wave * ldh + k] * W[o * K + k];
+  __shared__ float Ws[4096];
+  for (int i = threadIdx.x; i < O * K; i += blockDim.x) Ws[i] = W[i];
+  __syncthreads();
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  const __bf16 *row = H + m * ldh;
+  for (int k = 0; k < K; k += 8) {
+    const bf16x8 v = *reinterpret_cast<const bf16x8 *>(row + k);
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if (lane == 0) Y[(int64_t)wave * O + o] = s + ((wave < rows_with_bias && b) ? b[o] : 0.0f);
+    for (int q = 0; q < 8; ++q) {
+      const float x = (float)v[q];
+#pragma unroll
+      for (int o = 0; o < 4; ++o)
+        if (o < O) acc[o] = fmaf(x, Ws[o * K + k + q], acc[o]);
+    }
   }
+#pragma unroll
+  for (int o = 0; o < 4; ++o)
+    if (o < O) Y[m * O + o] = acc[o] + ((m < rows_with_bias && b) ? b[o] : 0.0f);
 }
 
 // Backward of the output layer for one row: dH[m, k] = sum_o dY[m, o] W[o, k]
@@ -296,8 +366,8 @@ int pyftle_mlp_gemm(const void *A, int64_t lda, int32_t a_k_contig, const void *
                     void *stream) {
   if (!A || !B || !C) return fail("mlp_gemm: NULL operand");
   if (M <= 0 || N <= 0 || K <= 0) return fail("mlp_gemm: sizes must be positive");
-  if (M % BM || N % BN || K % BK) return fail("mlp_gemm: M, N must be multiples of 128 and K of 32");
-  if (splits < 1 || K % (splits * BK)) return fail("mlp_gemm: K must split into multiples of 32");
+  if (M % BM || N % BN || K % BK) return fail("mlp_gemm: M, N must be multiples of 128 and K of 64");
+  if (splits < 1 || K % (splits * BK)) return fail("mlp_gemm: K must split into multiples of 64");
   if (splits > 1 && epilogue != EPI_F32) return fail("mlp_gemm: split-K needs the FP32 epilogue");
   if ((epilogue == EPI_BIAS_SILU && !bias) || (epilogue == EPI_DSILU && !Z))
     return fail("mlp_gemm: epilogue operand missing");
@@ -343,9 +413,10 @@ int pyftle_mlp_fourier(const float *in, int32_t d, const float *scale, const flo
 int pyftle_mlp_linear_out(const void *H, int64_t ldh, int32_t K, const float *W, const float *b,
                           int32_t O, int32_t M, int32_t rows_with_bias, float *Y, void *stream) {
   if (!H || !W || !Y) return fail("mlp_linear_out: NULL pointer");
-  if (K < 1 || O < 1 || M < 0 || ldh < K) return fail("mlp_linear_out: bad sizes");
+  if (K < 1 || O < 1 || O > 4 || O * K > 4096 || K % 8 || ldh % 8 || M < 0 || ldh < K)
+    return fail("mlp_linear_out: needs O <= 4, O*K <= 4096, K and ldh multiples of 8");
   if (M == 0) return PYFTLE_OK;
-  linear_out_kernel<<<(unsigned)((M + 3) / 4), 256, 0, as_stream(stream)>>>(
+  linear_out_kernel<<<(unsigned)((M + 255) / 256), 256, 0, as_stream(stream)>>>(
       (const __bf16 *)H, ldh, K, W, b, O, M, rows_with_bias, Y);
   return check_launch("mlp_linear_out");
 }

@@ -43,6 +43,7 @@ _SIGS = {
     "pyftle_mlp_cast_bf16": [_vp, _i64, _vp, _vp],
 }
 _registered = False
+_WORKSPACE: dict = {}
 
 
 def _lib():
@@ -170,12 +171,14 @@ class FlowMapMLP:
 
     # -- workspace ---------------------------------------------------------
     def _workspace(self, Mp: int, blocks: int):
-        key = (Mp, blocks)
-        ws = self._ws.get(key)
+        """Activation buffers, shared by every network of the same shape (one size resident)."""
+        s = self.spec
+        key = (s.d_in, s.d_out, s.n_freq, s.width, s.depth, Mp, blocks, str(self.params.device))
+        ws = _WORKSPACE.get(key)
         if ws is None:
-            s = self.spec
             dev = self.params.device
             R = Mp * blocks
+            _WORKSPACE.clear()
             ws = {
                 "x": torch.zeros(Mp, s.d_in, dtype=torch.float32, device=dev),
                 "F": torch.empty(R, 2 * s.n_freq, dtype=torch.bfloat16, device=dev),
@@ -183,7 +186,8 @@ class FlowMapMLP:
                 "Z": [torch.empty(Mp, s.width, dtype=torch.bfloat16, device=dev) for _ in range(s.depth)],
                 "Y": torch.empty(R, s.d_out, dtype=torch.float32, device=dev),
             }
-            self._ws = {key: ws}  # keep one workspace size resident
+            _WORKSPACE[key] = ws
+        self._ws = {(Mp, blocks): ws}
         return ws
 
     # -- forward / JVP -----------------------------------------------------
@@ -237,7 +241,7 @@ class FlowMapMLP:
         s = self.spec
         ws = self._ws[(Mp, 1)]
         dev = dY.device
-        chunk = 4096
+        chunk = 256  # rows per partial sum of the narrow reductions
         chunks = (Mp + chunk - 1) // chunk
         g = self.grads
         o, shp, ob = self.out_off
@@ -249,8 +253,8 @@ class FlowMapMLP:
                _p(ws["Z"][s.depth - 1]), s.width, chunk, _p(dWp), _p(dbp), _p(dZ), stream_handle())
         N.call("pyftle_mlp_reduce_splits", _p(dWp), chunks, s.d_out * s.width, _off(g, o), stream_handle())
         N.call("pyftle_mlp_reduce_splits", _p(dbp), chunks, s.d_out, _off(g, ob), stream_handle())
-        splits = max(1, Mp // chunk)
-        while Mp % (splits * 32):
+        splits = max(1, Mp // 4096)
+        while Mp % (splits * 64):
             splits -= 1
         F2 = 2 * s.n_freq
         for l in range(s.depth - 1, -1, -1):

@@ -41,7 +41,7 @@ def test_gemm_layouts(a_kc, b_kc):
     from pyftle_amd.surrogate import EPI_F32, _p, gemm
 
     g = torch.Generator(device="cuda").manual_seed(1)
-    M, N, K = 256, 384, 160
+    M, N, K = 256, 384, 192
     A = torch.randn(M, K, device="cuda", generator=g)
     B = torch.randn(K, N, device="cuda", generator=g)
     Ab = _bf(A if a_kc else A.t())

@@ -1,0 +1,113 @@
+"""GPU: the learned flow-map surrogate against plain RK4 (its fallback and judge).
+
+* trained on RK4 trajectories of a small double gyre, the ensemble's FTLE is
+  within 10% (RMSE / range) of RK4 and gains over the method without it
+  (plain RK4 on the same seeds) in time per field; the full-size measurement is
+  ``bench.py --workload surrogate2d``;
+* a member perturbed on purpose disagrees everywhere, so every seed falls back
+  and the field equals the RK4 field exactly;
+* on a linear flow the JVP Jacobian of a network fitted to the exact
+  displacement gives the closed-form FTLE.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _lib():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from pyftle_amd import surrogate
+
+    surrogate._lib()
+
+
+@pytest.fixture(scope="module")
+def gyre_setup():
+    from pyftle_amd import _native as N
+    from pyftle_amd import surrogate as S
+    from pyftle_amd import surrogate_ftle as SF
+    from pyftle_amd.field import DeviceField, GridGeometry
+
+    vels, pts = H.double_gyre_window(nx=128, ny=64, n_snap=41, snap_dt=0.1)
+    f = DeviceField(GridGeometry.from_reference((128, 64), pts), dtype="float32", layout=N.LAYOUT_PAIR,
+                    ring_slots=41, n_snapshots=41, t0=0.0, dt_snap=0.1)
+    for k, v in enumerate(vels):
+        f.pack(k, v)
+    T, h = 2.0, 0.05
+    X, D = SF.make_training_data(f, 262144, T, h, (0.0, 2.0), (0.05, 0.05), (1.95, 0.95), n_bins=64)
+    spec = S.MLPSpec(sigma=1.0, width=128, depth=3, scale=(0.5, 1.0, 0.5))
+    ens = SF.fit_ensemble(X, D, spec, members=2, epochs=25, batch=8192, lr=3e-3)
+    return f, ens, T, h
+
+
+def _seeds(nsx=128, nsy=64):
+    xs = (torch.arange(nsx, device="cuda", dtype=torch.float64) + 0.5) * (1.8 / nsx) + 0.1
+    ys = (torch.arange(nsy, device="cuda", dtype=torch.float64) + 0.5) * (0.8 / nsy) + 0.1
+    cy, cx = torch.meshgrid(ys, xs, indexing="ij")
+    return torch.stack([cx.reshape(-1), cy.reshape(-1)], 1).contiguous()
+
+
+def test_surrogate_ftle_close_to_rk4(gyre_setup):
+    from pyftle_amd import surrogate_ftle as SF
+
+    f, ens, T, h = gyre_setup
+    sf = SF.SurrogateFTLE(ens, f, T, h, tol=1e9)  # no fallback: the network alone
+    seeds = _seeds()
+    for t0 in (0.3, 1.4):
+        res = sf.ftle(seeds, t0)
+        ref = sf.rk4_ftle(seeds, t0)
+        err = SF.relative_rmse(res.ftle, ref)
+        print(f"[gpu] t0={t0} surrogate FTLE rmse/range {err:.4f}", flush=True)
+        assert err < 0.10
+        assert not bool(res.fallback.any())
+
+
+def test_fallback_recomputes_disagreeing_seeds_with_rk4(gyre_setup):
+    from pyftle_amd import surrogate_ftle as SF
+    from pyftle_amd.surrogate import FlowMapEnsemble
+
+    f, ens, T, h = gyre_setup
+    bad = ens.members[1]
+    saved = bad.params.clone()
+    o, shp, ob = bad.out_off
+    bad.params[ob:ob + shp[0]] += 0.5  # shift one member's output: it disagrees everywhere
+    bad.sync_bf16()
+    try:
+        sf = SF.SurrogateFTLE(FlowMapEnsemble(ens.members), f, T, h, tol=1e-3)
+        seeds = _seeds(64, 32)
+        res = sf.ftle(seeds, 0.7)
+        assert bool(res.fallback.all())
+        assert torch.equal(res.ftle, sf.rk4_ftle(seeds, 0.7))
+    finally:
+        bad.params.copy_(saved)
+        bad.sync_bf16()
+
+
+def test_jvp_ftle_on_linear_flow():
+    """Saddle u = l x, v = -l y: displacement d(x) = (e^{lT}-1) x, (e^{-lT}-1) y exactly."""
+    from pyftle_amd import surrogate as S
+
+    lam, T = 0.4, 1.0
+    spec = S.MLPSpec(d_in=3, sigma=0.5, width=128, depth=2, scale=(1.0, 1.0, 1.0))
+    m = S.FlowMapMLP(spec, seed=5)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    X = torch.rand(65536, 3, device="cuda", generator=g) - 0.5
+    X[:, 2] = 0.0
+    D = torch.stack([(math.exp(lam * T) - 1) * X[:, 0], (math.exp(-lam * T) - 1) * X[:, 1]], 1)
+    S.train(m, X, D, epochs=40, batch=8192, lr=3e-3)
+    q = torch.rand(4096, 3, device="cuda", generator=g) * 0.6 - 0.3
+    q[:, 2] = 0.0
+    _, Jd, _ = S.FlowMapEnsemble([m]).predict(q, 2)
+    ftle = S.ftle_from_displacement_jacobian(Jd, T).cpu().numpy()
+    print(f"[gpu] linear-flow FTLE mean {ftle.mean():.4f} (exact {lam})", flush=True)
+    assert abs(np.median(ftle) - lam) < 0.02
