@@ -40,9 +40,15 @@ struct GemmArgs {
   int64_t ldc;
 };
 
-__device__ __forceinline__ float silu_f(float z) { return z / (1.0f + __expf(-z)); }
+// Logistic sigmoid with the hardware exp2 and reciprocal (about 1 ulp each; results are
+// rounded to bf16 afterwards).  e^-z overflows to +inf for z < -88 and the reciprocal
+// then gives 0, the correct limit.
+__device__ __forceinline__ float sigmoid_hw(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * -1.4426950408889634f));
+}
+__device__ __forceinline__ float silu_f(float z) { return z * sigmoid_hw(z); }
 __device__ __forceinline__ float dsilu_f(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigmoid_hw(z);
   return s * (1.0f + z * (1.0f - s));
 }
 

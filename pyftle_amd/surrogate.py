@@ -41,6 +41,9 @@ _SIGS = {
     "pyftle_mlp_mse": [_vp, _vp, _i64, _f32, _vp, _vp, _vp],
     "pyftle_mlp_adam": [_vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _i32, _f32, _vp, _vp],
     "pyftle_mlp_cast_bf16": [_vp, _i64, _vp, _vp],
+    "pyftle_mlp_pack_fused": [_vp, _i32, _i32, _i32, _vp, _vp],
+    "pyftle_mlp_fused_forward": [_vp, _i32, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _f32,
+                                 _i32, _vp, _vp],
 }
 _registered = False
 _WORKSPACE: dict = {}
@@ -119,6 +122,8 @@ class FlowMapMLP:
         self.params[o:o + wout.numel()] = wout.reshape(-1).float().to(dev)
         self.out_off = (o, (spec.d_out, spec.width), o + wout.numel())
         self.params_bf16 = torch.empty(self.n_params, dtype=torch.bfloat16, device=dev)
+        self._fused = None
+        self._fused_step = -1
         self.sync_bf16()
         self.m = torch.zeros_like(self.params)
         self.v = torch.zeros_like(self.params)
@@ -148,7 +153,9 @@ class FlowMapMLP:
         return self.params[ob:ob + shp[0]]
 
     def sync_bf16(self):
+        """Refresh the BF16 copies after the FP32 parameters changed outside ``adam_step``."""
         N.call("pyftle_mlp_cast_bf16", _p(self.params), self.n_params, _p(self.params_bf16), stream_handle())
+        self._fused = None
 
     def to_numpy(self):
         """Parameters as the oracle's dict (FP64 copies of the BF16 GEMM weights)."""
@@ -161,6 +168,43 @@ class FlowMapMLP:
             "bout": self.bout().double().cpu().numpy() * s.out_scale,
         }
 
+    def fused_supported(self, n_tangents: int) -> bool:
+        s = self.spec
+        return (s.n_freq == 64 and s.width == 256 and 2 <= s.depth <= 8 and s.d_in <= 4
+                and (s.d_out, n_tangents) in ((2, 0), (2, 2), (3, 0)))
+
+    def pack_fused(self):
+        """Weights in the fused kernel's order (bf16) and the hidden biases (depth, width)."""
+        s = self.spec
+        n = sum(shp[0] * shp[1] for _, shp, _ in self.offsets)
+        Wp = torch.empty(n, dtype=torch.bfloat16, device=self.params.device)
+        pos = 0
+        for l, (o, (rows, k), _) in enumerate(self.offsets):
+            N.call("pyftle_mlp_pack_fused", _off(self.params, o), rows, k, int(l > 0), _off(Wp, pos),
+                   stream_handle())
+            pos += rows * k
+        bias = torch.stack([self.b(l) for l in range(s.depth)]).contiguous()
+        self._fused = (Wp, bias, self.Wout().contiguous(), self.bout().contiguous())
+        self._fused_step = self.step
+        return self._fused
+
+    def forward_fused(self, x: torch.Tensor, n_tangents: int = 0, out: torch.Tensor | None = None):
+        """Inference through the one-kernel network; ``(1 + n_tangents, M, d_out)`` FP32."""
+        s = self.spec
+        if not self.fused_supported(n_tangents):
+            raise ValueError(f"fused inference does not cover {s} with {n_tangents} tangents")
+        if getattr(self, "_fused", None) is None or self._fused_step != self.step:
+            self.pack_fused()
+        Wp, bias, wout, bout = self._fused
+        x = x.contiguous()
+        M = x.shape[0]
+        Y = out if out is not None else torch.empty(1 + n_tangents, M, s.d_out, dtype=torch.float32,
+                                                    device=x.device)
+        N.call("pyftle_mlp_fused_forward", _p(x), s.d_in, M, _p(self.scale), _p(self.B), s.n_freq, _p(Wp),
+               _p(bias), s.depth, s.width, _p(wout), _p(bout), s.d_out, float(s.out_scale), int(n_tangents),
+               _p(Y), stream_handle())
+        return Y
+
     def state_dict(self):
         return {"spec": self.spec.__dict__, "B": self.B.cpu(), "params": self.params.cpu()}
 
@@ -168,6 +212,7 @@ class FlowMapMLP:
         self.B.copy_(sd["B"].to(self.B.device))
         self.params.copy_(sd["params"].to(self.params.device))
         self.sync_bf16()
+        self._fused = None
 
     # -- workspace ---------------------------------------------------------
     def _workspace(self, Mp: int, blocks: int):
@@ -310,13 +355,22 @@ class FlowMapEnsemble:
     def __init__(self, members: list[FlowMapMLP]):
         self.members = members
 
-    def predict(self, x: torch.Tensor, dim: int):
-        """Mean displacement, mean Jacobian of the displacement and member spread."""
+    def predict(self, x: torch.Tensor, dim: int, fused: bool = True):
+        """Mean displacement, mean Jacobian of the displacement and member spread.
+
+        ``fused`` runs the one-kernel network where it covers the shape; the
+        layer-by-layer kernels otherwise.
+        """
         outs, jacs = [], []
         for m in self.members:
-            y, tans = m.forward(x, n_tangents=dim)
-            outs.append(y.clone())
-            jacs.append(torch.stack([t.clone() for t in tans], dim=2))  # (M, d_out, dim): d y_r / d x_c
+            if fused and m.fused_supported(dim):
+                Y = m.forward_fused(x, n_tangents=dim)
+                outs.append(Y[0])
+                jacs.append(Y[1:].permute(1, 2, 0))  # (M, d_out, dim): d y_r / d x_c
+            else:
+                y, tans = m.forward(x, n_tangents=dim)
+                outs.append(y.clone())
+                jacs.append(torch.stack([t.clone() for t in tans], dim=2))
         Y = torch.stack(outs)
         Jd = torch.stack(jacs)
         mean = Y.mean(0)

@@ -153,3 +153,57 @@ def test_training_reduces_loss():
     losses = train(m, X, T, epochs=15, batch=2048, lr=3e-3)
     print(f"[gpu] loss {losses[0]:.3e} -> {losses[-1]:.3e}", flush=True)
     assert losses[-1] < 0.1 * losses[0]
+
+
+def _fused_model(seed=4, depth=3, d_out=2):
+    from pyftle_amd.surrogate import FlowMapMLP, MLPSpec
+
+    spec = MLPSpec(d_in=3, d_out=d_out, n_freq=64, width=256, depth=depth, sigma=1.5, scale=(0.5, 1.0, 0.1),
+                   out_scale=0.7)
+    return FlowMapMLP(spec, seed=seed)
+
+
+@pytest.mark.parametrize("M", [1, 37, 300, 4096])
+def test_fused_forward_and_jvp_match_oracle(M):
+    """One-kernel network (weights permuted for the register-resident layout) vs FP64."""
+    m = _fused_model()
+    p = m.to_numpy()
+    rng = np.random.default_rng(M)
+    x = rng.uniform([0, 0, 0], [2, 1, 10], (M, 3))
+    Y = m.forward_fused(torch.as_tensor(x, dtype=torch.float32, device="cuda"), n_tangents=2).cpu().numpy()
+    assert Y.shape == (3, M, 2)
+    assert _rel(Y[0], MO.forward(p, x)) < 3e-2
+    for a in range(2):
+        assert _rel(Y[1 + a], MO.jvp(p, x, a)) < 3e-2
+    Y0 = m.forward_fused(torch.as_tensor(x, dtype=torch.float32, device="cuda")).cpu().numpy()
+    assert np.array_equal(Y0[0], Y[0])  # the primal rows do not depend on the tangents
+
+
+def test_fused_matches_layered_path():
+    m = _fused_model(seed=5, depth=5)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.rand(1000, 3, device="cuda", generator=g) * torch.tensor([2.0, 1.0, 10.0], device="cuda")
+    y, tans = m.forward(x, n_tangents=2)
+    Y = m.forward_fused(x, n_tangents=2)
+    assert _rel(Y[0].cpu().numpy(), y.cpu().numpy()) < 2e-2
+    for a in range(2):
+        assert _rel(Y[1 + a].cpu().numpy(), tans[a].cpu().numpy()) < 2e-2
+
+
+def test_fused_three_outputs_and_repack_after_update():
+    m = _fused_model(seed=6, depth=4, d_out=3)
+    p = m.to_numpy()
+    rng = np.random.default_rng(9)
+    x = rng.uniform([0, 0, 0], [2, 1, 10], (200, 3))
+    xt = torch.as_tensor(x, dtype=torch.float32, device="cuda")
+    assert _rel(m.forward_fused(xt)[0].cpu().numpy(), MO.forward(p, x)) < 3e-2
+    m.grads.copy_(torch.randn_like(m.grads))
+    m.adam_step(1e-2)  # weights change: the packed copy must follow
+    p2 = m.to_numpy()
+    assert _rel(m.forward_fused(xt)[0].cpu().numpy(), MO.forward(p2, x)) < 3e-2
+
+
+def test_fused_rejects_unsupported_shapes():
+    m = _small_model()  # width 128
+    with pytest.raises(ValueError, match="fused inference"):
+        m.forward_fused(torch.zeros(4, 3, device="cuda"))
