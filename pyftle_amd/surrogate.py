@@ -355,22 +355,31 @@ class FlowMapEnsemble:
     def __init__(self, members: list[FlowMapMLP]):
         self.members = members
 
-    def predict(self, x: torch.Tensor, dim: int, fused: bool = True):
+    def predict(self, x: torch.Tensor, dim: int, fused: bool = True, jacobian_members: int | None = None):
         """Mean displacement, mean Jacobian of the displacement and member spread.
 
-        ``fused`` runs the one-kernel network where it covers the shape; the
-        layer-by-layer kernels otherwise.
+        The spread is the largest deviation of a member's displacement from the
+        ensemble mean (any component).  ``jacobian_members`` (default: all) members
+        also push ``dim`` tangents through the network and their Jacobians are
+        averaged; the others run the primal rows only.  ``fused`` runs the
+        one-kernel network where it covers the shape, the layer-by-layer kernels
+        otherwise.
         """
+        nj = len(self.members) if jacobian_members is None else max(1, min(jacobian_members, len(self.members)))
         outs, jacs = [], []
-        for m in self.members:
-            if fused and m.fused_supported(dim):
-                Y = m.forward_fused(x, n_tangents=dim)
+        for k, m in enumerate(self.members):
+            nt = dim if k < nj else 0
+            if fused and m.fused_supported(nt):
+                Y = m.forward_fused(x, n_tangents=nt)
                 outs.append(Y[0])
-                jacs.append(Y[1:].permute(1, 2, 0))  # (M, d_out, dim): d y_r / d x_c
-            else:
-                y, tans = m.forward(x, n_tangents=dim)
+                if nt:
+                    jacs.append(Y[1:].permute(1, 2, 0))  # (M, d_out, dim): d y_r / d x_c
+            elif nt:
+                y, tans = m.forward(x, n_tangents=nt)
                 outs.append(y.clone())
                 jacs.append(torch.stack([t.clone() for t in tans], dim=2))
+            else:
+                outs.append(m.forward(x).clone())
         Y = torch.stack(outs)
         Jd = torch.stack(jacs)
         mean = Y.mean(0)

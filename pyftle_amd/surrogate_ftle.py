@@ -84,8 +84,10 @@ class SurrogateFTLE:
     """FTLE of seeds at window start t0 from the ensemble, RK4 where it disagrees."""
 
     def __init__(self, ensemble: FlowMapEnsemble, field: DeviceField, T: float, h: float,
-                 delta: float = 1e-3, tol: float = 1e-3, time_mode: str = "linear", chunk: int = 1 << 20):
+                 delta: float = 1e-3, tol: float = 1e-3, time_mode: str = "linear", chunk: int = 1 << 21,
+                 jacobian_members: int | None = None):
         self.ens = ensemble
+        self.jacobian_members = jacobian_members
         self.field = field
         self.T = T
         self.h = h
@@ -103,7 +105,7 @@ class SurrogateFTLE:
             x = torch.empty(sd.shape[0], dim + 1, dtype=torch.float32, device=sd.device)
             x[:, :dim] = sd.float()
             x[:, dim] = t0
-            _, Jd, spread = self.ens.predict(x, dim)
+            _, Jd, spread = self.ens.predict(x, dim, jacobian_members=self.jacobian_members)
             out_f.append(ftle_from_displacement_jacobian(Jd, self.T))
             out_s.append(spread)
         return torch.cat(out_f), torch.cat(out_s)
