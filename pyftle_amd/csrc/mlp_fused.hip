@@ -17,9 +17,12 @@
 // that order).  Tangent rows share the weight fragment: each A fragment read
 // from LDS feeds 1 + n_tangents MFMAs.
 //
-// Work-group: 4 waves x 16 seeds per tile, persistent over tiles.  Weights stream
-// through LDS in chunks of 128 output features, double-buffered: the next chunk is
-// copied by direct-to-LDS loads while the current one is multiplied.  The output layer (O <= 3) runs in
+// Work-group: 4 waves x 16 seeds per tile (x 3 seed groups without tangents),
+// persistent over tiles.  Weights stream through LDS in chunks of 128 output features,
+// double-buffered: the next chunk is copied by direct-to-LDS loads, spread over the
+// k-steps, while the current one is multiplied.  Each chunk is multiplied as two
+// 64-feature halves with alternating accumulator sets, so the SiLU epilogue of one half
+// is issued between the MFMAs of the next.  The output layer (O <= 3) runs in
 // FP32 on the last hidden layer's registers, reduced across the 4 lanes of a
 // seed with cross-lane shuffles.
 
@@ -75,7 +78,8 @@ __device__ __forceinline__ int64_t layer_offset(int L) {
 // banks.  Row R % 16 of instruction i depends only on i and the lane, so the per-lane
 // offsets repeat with period 16 / rows-per-instruction and the rest is a scalar offset.
 template <int K>
-__device__ __forceinline__ void chunk_dma(__amdgpu_buffer_rsrc_t w, int c, __bf16 *buf, int wave, int lane) {
+__device__ __forceinline__ void chunk_dma(__amdgpu_buffer_rsrc_t w, int c, __bf16 *buf, int wave, int lane,
+                                          int i0 = 0, int n = 64) {
   constexpr int kUnits = K / 8;              // 16-byte units per row
   constexpr int kRowsPerInst = 64 / kUnits;  // rows per wave instruction (1 KB)
   constexpr int kInstPerWave = kChunk / kRowsPerInst / kWaves;
@@ -84,6 +88,7 @@ __device__ __forceinline__ void chunk_dma(__amdgpu_buffer_rsrc_t w, int c, __bf1
   const int rr = lane / kUnits, p = lane % kUnits;
 #pragma unroll
   for (int i = 0; i < kInstPerWave; ++i) {
+    if (i < i0 || i >= i0 + n) continue;
     const int blk = wave * kInstPerWave + i;  // 1 KB block of the chunk
     const int voff = rr * K * 2 + (p ^ (kRowsPerInst * (i % kPatterns) + rr)) * 16;
     __builtin_amdgcn_raw_ptr_buffer_load_lds(w, (__attribute__((address_space(3))) void *)(buf + blk * 512), 16,
@@ -91,9 +96,16 @@ __device__ __forceinline__ void chunk_dma(__amdgpu_buffer_rsrc_t w, int c, __bf1
   }
 }
 
-__device__ __forceinline__ void chunk_dma_any(__amdgpu_buffer_rsrc_t w, int c, __bf16 *buf, int wave, int lane) {
-  if (c < 2) chunk_dma<kFeat>(w, c, buf, wave, lane);
-  else chunk_dma<kWidth>(w, c, buf, wave, lane);
+// Part `part` of `parts` of chunk c's copy (the instructions of one wave split evenly).
+__device__ __forceinline__ void chunk_dma_part(__amdgpu_buffer_rsrc_t w, int c, __bf16 *buf, int wave, int lane,
+                                               int part, int parts) {
+  if (c < 2) {
+    const int per = (kChunk / (64 / (kFeat / 8)) / kWaves) / parts;
+    chunk_dma<kFeat>(w, c, buf, wave, lane, part * per, per);
+  } else {
+    const int per = (kChunk / (64 / (kWidth / 8)) / kWaves) / parts;
+    chunk_dma<kWidth>(w, c, buf, wave, lane, part * per, per);
+  }
 }
 
 // Wait for this wave's outstanding global loads (including the direct-to-LDS copies),
@@ -103,37 +115,44 @@ __device__ __forceinline__ void wait_loads_and_sync() {
   __syncthreads();
 }
 
-// acc[b][t] = W_chunk[t*16 .. +16][0..K) x in[b]  (16 seeds, all K input features).
-// The 8 weight fragments of k-step ks+1 are read from LDS before the MFMAs of k-step ks
-// are issued (the scheduling barrier keeps that order), so the LDS latency is covered by
-// 8 * NB MFMAs.
-template <int K, int NB>
-__device__ __forceinline__ void chunk_mma(const __bf16 *buf, const bf16x8_t (&in)[NB][8],
-                                          f32x4_t (&acc)[NB][8], int lane) {
+// acc[b][t] = W_chunk rows row0 + 16 t .. (t < 4: one sub-chunk of 64 output features) x
+// in[b] over all K input features (16 seeds per row block).  The 4 weight fragments of
+// k-step ks+1 are read from LDS before the MFMAs of k-step ks.  Piece ks of the previous
+// sub-chunk's epilogue, epi(ks), is issued between the MFMAs of k-step ks: it reads only
+// the other accumulator set, so its VALU work runs while the matrix core is busy.
+template <int K, int NB, typename Epi>
+__device__ __forceinline__ void sub_mma(const __bf16 *buf, int row0, const bf16x8_t (&in)[NB][8],
+                                        f32x4_t (&acc)[NB][4], int lane, Epi &&epi) {
   const int r = lane & 15, g = lane >> 4;
-  const __bf16 *base = buf + r * K;
-  const int sw = g ^ r;  // unit ks*4 + g of row t*16 + r sits at unit (ks*4) ^ sw
+  const __bf16 *base = buf + (row0 + r) * K;
+  const int sw = g ^ r;  // unit ks*4 + g of row row0 + 16 t + r sits at unit (ks*4) ^ sw
 #pragma unroll
   for (int b = 0; b < NB; ++b)
 #pragma unroll
-    for (int t = 0; t < 8; ++t) acc[b][t] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
-  bf16x8_t frag[2][8];
+    for (int t = 0; t < 4; ++t) acc[b][t] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+  bf16x8_t frag[2][4];
 #pragma unroll
-  for (int t = 0; t < 8; ++t) frag[0][t] = *reinterpret_cast<const bf16x8_t *>(base + t * 16 * K + sw * 8);
+  for (int t = 0; t < 4; ++t) frag[0][t] = *reinterpret_cast<const bf16x8_t *>(base + t * 16 * K + sw * 8);
 #pragma unroll
   for (int ks = 0; ks < K / 32; ++ks) {
     if (ks + 1 < K / 32) {
 #pragma unroll
-      for (int t = 0; t < 8; ++t)
+      for (int t = 0; t < 4; ++t)
         frag[(ks + 1) & 1][t] =
             *reinterpret_cast<const bf16x8_t *>(base + t * 16 * K + (((ks + 1) * 4) ^ sw) * 8);
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int t = 0; t < 8; ++t)
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int b = 0; b < NB; ++b)
         acc[b][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(frag[ks & 1][t], in[b][ks], acc[b][t], 0, 0, 0);
+    epi(ks);
+#pragma unroll
+    for (int i = 0; i < 4 * NB; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+      __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);  // then up to 4 VALU
+    }
     __builtin_amdgcn_sched_barrier(0);
   }
 }
@@ -145,67 +164,105 @@ __device__ __forceinline__ float sigmoid_f(float z) {
   return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * -1.4426950408889634f));
 }
 
+// Epilogues, one piece at a time: piece p of a sub-chunk covers elements
+// [p*EP, p*EP + EP) of its 16 per-lane values (tile t = p*EP / 4), EP = 16 / pieces.
 // Row blocks of a wave: G seed groups of 16, each followed by its NT tangent blocks
 // (block q * (1 + NT) + 1 + a holds d/d x_a of group q).
-// Hidden epilogue of chunk mc: h = SiLU(z + bias), tangents * SiLU'(z), packed as the next
-// layer's B operand (k-step mc*4 + t/2, element (t%2)*4 + e).
-template <int G, int NT>
-__device__ __forceinline__ void hidden_epilogue(const f32x4_t (&acc)[G * (1 + NT)][8], const float *bias_l,
-                                                int mc, bf16x8_t (&out)[G * (1 + NT)][8], int lane) {
+//
+// Hidden layer: h = SiLU(z + bias), tangents * SiLU'(z), packed as the next layer's B
+// operand: output feature mc*128 + tt*16 + 4g + e (tt = 4 h + t) goes to k-step
+// mc*4 + tt/2, element (tt%2)*4 + e.
+template <int G, int NT, int PIECES>
+__device__ __forceinline__ void hidden_piece(const f32x4_t (&acc)[G * (1 + NT)][4], const float *bias_l,
+                                             int mc, int h, int p, bf16x8_t (&out)[G * (1 + NT)][8], int lane) {
+  constexpr int EP = 16 / PIECES;
   const int g = lane >> 4;
+  const int t = (p * EP) >> 2, e0 = (p * EP) & 3, tt = h * 4 + t;
+  const f32x4_t bv = *reinterpret_cast<const f32x4_t *>(bias_l + mc * kChunk + tt * 16 + 4 * g);
+  const int ks = mc * 4 + (tt >> 1);
 #pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    const f32x4_t bv = *reinterpret_cast<const f32x4_t *>(bias_l + mc * kChunk + t * 16 + 4 * g);
+  for (int q = 0; q < G; ++q) {
+    const int b0 = q * (1 + NT);
 #pragma unroll
-    for (int q = 0; q < G; ++q) {
-      const int b0 = q * (1 + NT);
+    for (int e = e0; e < e0 + EP; ++e) {
+      const float z = acc[b0][t][e] + bv[e];
+      const float s = sigmoid_f(z);
+      const int j = (tt & 1) * 4 + e;
+      out[b0][ks][j] = (__bf16)(z * s);
+      if (NT > 0) {
+        const float d = s * (1.0f + z * (1.0f - s));
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const float z = acc[b0][t][e] + bv[e];
-        const float s = sigmoid_f(z);
-        const int ks = mc * 4 + (t >> 1), j = (t & 1) * 4 + e;
-        out[b0][ks][j] = (__bf16)(z * s);
-        if (NT > 0) {
-          const float d = s * (1.0f + z * (1.0f - s));
-#pragma unroll
-          for (int a = 1; a <= NT; ++a) out[b0 + a][ks][j] = (__bf16)(acc[b0 + a][t][e] * d);
-        }
+        for (int a = 1; a <= NT; ++a) out[b0 + a][ks][j] = (__bf16)(acc[b0 + a][t][e] * d);
       }
     }
   }
 }
 
 // Last hidden layer: the output layer's partial sums over this lane's features.
-template <int G, int NT, int O>
-__device__ __forceinline__ void output_epilogue(const f32x4_t (&acc)[G * (1 + NT)][8], const float *bias_l,
-                                                const float *wout, int mc, float (&yp)[G * (1 + NT)][O],
-                                                int lane) {
+template <int G, int NT, int O, int PIECES>
+__device__ __forceinline__ void output_piece(const f32x4_t (&acc)[G * (1 + NT)][4], const float *bias_l,
+                                             const float *wout, int mc, int h, int p,
+                                             float (&yp)[G * (1 + NT)][O], int lane) {
+  constexpr int EP = 16 / PIECES;
   const int g = lane >> 4;
+  const int t = (p * EP) >> 2, e0 = (p * EP) & 3, tt = h * 4 + t;
+  const int f0 = mc * kChunk + tt * 16 + 4 * g;
+  const f32x4_t bv = *reinterpret_cast<const f32x4_t *>(bias_l + f0);
+  f32x4_t wv[O];
 #pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    const int f0 = mc * kChunk + t * 16 + 4 * g;
-    const f32x4_t bv = *reinterpret_cast<const f32x4_t *>(bias_l + f0);
-    f32x4_t wv[O];
+  for (int o = 0; o < O; ++o) wv[o] = *reinterpret_cast<const f32x4_t *>(wout + o * kWidth + f0);
 #pragma unroll
-    for (int o = 0; o < O; ++o) wv[o] = *reinterpret_cast<const f32x4_t *>(wout + o * kWidth + f0);
+  for (int q = 0; q < G; ++q) {
+    const int b0 = q * (1 + NT);
 #pragma unroll
-    for (int q = 0; q < G; ++q) {
-      const int b0 = q * (1 + NT);
+    for (int e = e0; e < e0 + EP; ++e) {
+      const float z = acc[b0][t][e] + bv[e];
+      const float s = sigmoid_f(z);
+      const float hz = z * s;
+      const float d = s * (1.0f + z * (1.0f - s));
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const float z = acc[b0][t][e] + bv[e];
-        const float s = sigmoid_f(z);
-        const float h = z * s;
-        const float d = s * (1.0f + z * (1.0f - s));
+      for (int o = 0; o < O; ++o) {
+        yp[b0][o] = fmaf(hz, wv[o][e], yp[b0][o]);
 #pragma unroll
-        for (int o = 0; o < O; ++o) {
-          yp[b0][o] = fmaf(h, wv[o][e], yp[b0][o]);
-#pragma unroll
-          for (int a = 1; a <= NT; ++a) yp[b0 + a][o] = fmaf(acc[b0 + a][t][e] * d, wv[o][e], yp[b0 + a][o]);
-        }
+        for (int a = 1; a <= NT; ++a) yp[b0 + a][o] = fmaf(acc[b0 + a][t][e] * d, wv[o][e], yp[b0 + a][o]);
       }
     }
   }
+}
+
+// One layer (K input features) of one tile: chunk 2L (output features 0..127) from LDS
+// buffer 0, chunk 2L+1 (128..255) from buffer 1, each as two 64-feature sub-chunks with
+// the accumulators alternating between accA and accB.  The epilogue of each sub-chunk
+// runs under the MFMAs of the next one, except the last (the next layer needs all of this
+// layer's outputs).  The copy of chunk 2L+1 is issued before chunk 2L is multiplied and
+// the copy of the chunk after 2L+1 (the next layer's first, or chunk 0 of the next tile)
+// before chunk 2L+1.
+template <int K, int G, int NT, int O, bool LAST>
+__device__ __forceinline__ void run_layer(int L, int nchunks, __amdgpu_buffer_rsrc_t wr, __bf16 *wbuf0,
+                                          __bf16 *wbuf1, const float *bias_s, const float *wout_s,
+                                          const bf16x8_t (&in)[G * (1 + NT)][8],
+                                          bf16x8_t (&out)[G * (1 + NT)][8], float (&yp)[G * (1 + NT)][O],
+                                          f32x4_t (&accA)[G * (1 + NT)][4], f32x4_t (&accB)[G * (1 + NT)][4],
+                                          int wave, int lane) {
+  constexpr int NB = G * (1 + NT), P = K / 32;
+  const float *bl = bias_s + L * kWidth;
+  auto piece = [&](const f32x4_t(&acc)[NB][4], int mc, int h, int p) {
+    if constexpr (LAST) output_piece<G, NT, O, P>(acc, bl, wout_s, mc, h, p, yp, lane);
+    else hidden_piece<G, NT, P>(acc, bl, mc, h, p, out, lane);
+  };
+  // the copies are spread over the k-steps of the first sub-chunk of each chunk
+  const int c1 = 2 * L + 1, c2 = (2 * L + 2) % nchunks;
+  sub_mma<K, NB>(wbuf0, 0, in, accA, lane, [&](int p) { chunk_dma_part(wr, c1, wbuf1, wave, lane, p, P); });
+  sub_mma<K, NB>(wbuf0, 64, in, accB, lane, [&](int p) { piece(accA, 0, 0, p); });
+  wait_loads_and_sync();
+  sub_mma<K, NB>(wbuf1, 0, in, accA, lane, [&](int p) {
+    chunk_dma_part(wr, c2, wbuf0, wave, lane, p, P);
+    piece(accB, 0, 1, p);
+  });
+  sub_mma<K, NB>(wbuf1, 64, in, accB, lane, [&](int p) { piece(accA, 1, 0, p); });
+#pragma unroll
+  for (int p = 0; p < P; ++p) piece(accB, 1, 1, p);
+  wait_loads_and_sync();
 }
 
 // Persistent: work-group b handles seed tiles b, b + gridDim.x, ...  The weight chunks
@@ -305,7 +362,7 @@ __global__ void __launch_bounds__(kThreads, 1) fused_mlp_kernel(FusedArgs a) {
         xin[q][c] = (nseed < a.M && c < a.d_in) ? a.x[nseed * a.d_in + c] : 0.0f;
     }
 
-    f32x4_t acc[NB][8];
+    f32x4_t accA[NB][4], accB[NB][4];
     bf16x8_t out[NB][8];
     float yp[NB][O];
 #pragma unroll
@@ -313,32 +370,19 @@ __global__ void __launch_bounds__(kThreads, 1) fused_mlp_kernel(FusedArgs a) {
 #pragma unroll
       for (int o = 0; o < O; ++o) yp[b][o] = 0.0f;
 
-    // chunk c + 1 is copied into the other buffer while chunk c is multiplied;
-    // layer 0 (K = 128) is chunks 0 and 1
-    chunk_dma<kFeat>(wr, 1, wbuf1, wave, lane);
-    chunk_mma<kFeat, NB>(wbuf0, in, acc, lane);
-    hidden_epilogue<G, NT>(acc, bias_s, 0, out, lane);
-    wait_loads_and_sync();
-    chunk_dma<kWidth>(wr, 2, wbuf0, wave, lane);
-    chunk_mma<kFeat, NB>(wbuf1, in, acc, lane);
-    hidden_epilogue<G, NT>(acc, bias_s, 1, out, lane);
-    wait_loads_and_sync();
-
+    run_layer<kFeat, G, NT, O, false>(0, nchunks, wr, wbuf0, wbuf1, bias_s, wout_s, in, out, yp, accA, accB,
+                                      wave, lane);
     for (int L = 1; L < a.depth; ++L) {
 #pragma unroll
       for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int k = 0; k < 8; ++k) in[b][k] = out[b][k];
-      const bool last = L == a.depth - 1;
-#pragma unroll
-      for (int mc = 0; mc < 2; ++mc) {
-        const int c1 = (2 * L + mc + 1) % nchunks;  // chunk 0 of the next tile after the last
-        chunk_dma_any(wr, c1, mc ? wbuf0 : wbuf1, wave, lane);
-        chunk_mma<kWidth, NB>(mc ? wbuf1 : wbuf0, in, acc, lane);
-        if (last) output_epilogue<G, NT, O>(acc, bias_s + L * kWidth, wout_s, mc, yp, lane);
-        else hidden_epilogue<G, NT>(acc, bias_s + L * kWidth, mc, out, lane);
-        wait_loads_and_sync();
-      }
+      if (L == a.depth - 1)
+        run_layer<kWidth, G, NT, O, true>(L, nchunks, wr, wbuf0, wbuf1, bias_s, wout_s, in, out, yp, accA, accB,
+                                          wave, lane);
+      else
+        run_layer<kWidth, G, NT, O, false>(L, nchunks, wr, wbuf0, wbuf1, bias_s, wout_s, in, out, yp, accA,
+                                           accB, wave, lane);
     }
 
     // the 4 lanes of a seed (lane % 16 equal) hold disjoint feature subsets
