@@ -312,14 +312,115 @@ def run_surrogate(torch, wl, args):
                 "seed_steps_per_s": n_sw * round(wl["T"] / wl["h"]) / (rk4_ms * 1e-3)},
         "speedup_vs_gpu_rk4": rk4_ms / main["ms"],
         "surrogate_only_speedup_vs_gpu_rk4": rk4_ms / v["ms"],
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(_gyre_cpu_workload(torch, wl), budget_s=15.0),
     }
     print(json.dumps(result), flush=True)
     plan.close()
 
 
+def build_surrogate_infer(torch, nsx=2048, nsy=1024, n_t0=32, members=4):
+    """Inference half of surrogate2d: the same 67.1M seed-windows through four
+    random-init members of the same architecture (no training), for profiling the
+    fused MLP kernel."""
+    wl = build_surrogate2d(torch, nsx=nsx, nsy=nsy, n_t0=n_t0, members=members, epochs=0, n_traj=0)
+    wl["name"] = "surrogate_infer"
+    wl["config"] = dict(wl["config"], workload="surrogate_infer", weights="random init (no training)")
+    for k in ("epochs", "trajectories"):
+        wl["config"].pop(k, None)
+    return wl
+
+
+def _gyre_cpu_workload(torch, wl):
+    """The surrogate's flow as the reference CPU path sees it: snapshot 0 of the
+    512x256 gyre on the host and the seed-neighbour groups of a 400x200 seed grid
+    (one window, frozen field, h = 0.05, 200 steps)."""
+    import numpy as np
+
+    nx, ny = wl["config"]["grid"]
+    X, Y = np.meshgrid(np.linspace(0.0, 2.0, nx), np.linspace(0.0, 1.0, ny))
+    u, v = _double_gyre(0.0, X, Y)
+    nsx, nsy, delta = 400, 200, 1e-3
+    dev = torch.device("cuda")
+    xs = (torch.arange(nsx, device=dev, dtype=torch.float64) + 0.5) * (2.0 / nsx)
+    ys = (torch.arange(nsy, device=dev, dtype=torch.float64) + 0.5) * (1.0 / nsy)
+    cy, cx = torch.meshgrid(ys, xs, indexing="ij")
+    c = torch.stack([cx.reshape(-1), cy.reshape(-1)], dim=1)
+    ex = torch.tensor([delta, 0.0], dtype=torch.float64, device=dev)
+    ey = torch.tensor([0.0, delta], dtype=torch.float64, device=dev)
+    pos = torch.cat([c - ex, c + ex, c + ey, c - ey], dim=0).contiguous()
+    return dict(name="gyre512", pos=pos, n_seeds=nsx * nsy, dim=2, h=wl["h"], steps=int(round(wl["T"] / wl["h"])),
+                host=dict(snaps=[np.stack([u.ravel(), v.ravel()])], nx=nx, ny=ny))
+
+
+def run_surrogate_infer(torch, wl, args):
+    """Ensemble FTLE (no fallback) over all seed-windows, and the fused kernel alone
+    timed with HIP events on the current stream (the roofline entry)."""
+    from pyftle_amd import surrogate as S
+
+    SF = wl["SF"]
+    spec = S.MLPSpec(sigma=1.0)
+    ens = S.FlowMapEnsemble([S.FlowMapMLP(spec, seed=1000 + k) for k in range(wl["members"])])
+    nsx, nsy = wl["nsx"], wl["nsy"]
+    dev = torch.device("cuda")
+    xs = (torch.arange(nsx, device=dev, dtype=torch.float64) + 0.5) * (2.0 / nsx)
+    ys = (torch.arange(nsy, device=dev, dtype=torch.float64) + 0.5) * (1.0 / nsy)
+    cy, cx = torch.meshgrid(ys, xs, indexing="ij")
+    seeds = torch.stack([cx.reshape(-1), cy.reshape(-1)], 1).contiguous()
+    t0s = [10.0 * k / wl["n_t0"] for k in range(wl["n_t0"])]
+    n_sw = seeds.shape[0] * len(t0s)
+    sf = SF.SurrogateFTLE(ens, wl["field"], wl["T"], wl["h"], tol=float("inf"))
+    fpq = S.mlp_flops_per_query(spec)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    steps = max(1, args.steps)
+    out = {}
+    for jm in (wl["members"], 1):
+        sf.jacobian_members = jm
+        ms = timed(lambda: [sf.predict(seeds, t0) for t0 in t0s], steps)
+        rows = n_sw * (jm * 3 + (wl["members"] - jm))
+        out[jm] = {"ms": ms, "seeds_per_s": n_sw / (ms * 1e-3), "tflops": rows * fpq / (ms * 1e-3) / 1e12}
+        log(f"ensemble FTLE, Jacobian from {jm} member(s): {ms:.1f} ms, {out[jm]['tflops']:.0f} TFLOP/s")
+    # the kernel alone: one member, 2 tangents, 2M seeds per launch
+    m0 = ens.members[0]
+    x = torch.empty(seeds.shape[0], 3, dtype=torch.float32, device=dev)
+    x[:, :2] = seeds.float()
+    x[:, 2] = 2.5
+    Y = torch.empty(3, seeds.shape[0], 2, dtype=torch.float32, device=dev)
+    kms = timed(lambda: m0.forward_fused(x, n_tangents=2, out=Y), 10 * steps)
+    kflops = seeds.shape[0] * 3 * fpq
+    Y0 = torch.empty(1, seeds.shape[0], 2, dtype=torch.float32, device=dev)
+    kms0 = timed(lambda: m0.forward_fused(x, n_tangents=0, out=Y0), 10 * steps)
+    ach = kflops / (kms * 1e-3) / 1e12
+    main = out[wl["members"]]
+    cpu = None if args.no_cpu else cpu_baseline(_gyre_cpu_workload(torch, wl), budget_s=15.0)
+    result = {
+        "metric": METRIC, "value": main["seeds_per_s"], "unit": "seeds/s", "n_gpus": 1, "steps": steps,
+        "warmup": 1, "ms_per_step": main["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16", "data": "synthetic", "config": wl["config"],
+        "roofline": {"bound": "mfma", "achieved": ach, "peak": 2500.0, "unit": "TFLOP/s", "frac": ach / 2500.0,
+                     "traffic": None, "kernel": "fused_mlp_kernel<1,2,2> (2M seeds, primal + 2 tangent rows)",
+                     "flops_per_launch": kflops, "avg_launch_ms": kms},
+        "fused_primal_only": {"kernel": "fused_mlp_kernel<3,0,2>", "avg_launch_ms": kms0,
+                              "tflops": seeds.shape[0] * fpq / (kms0 * 1e-3) / 1e12},
+        "ensemble": {str(k): v for k, v in out.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+
+
 WORKLOADS = {"abc128": build_abc128, "gyre512": build_gyre, "batch64": build_batch64,
-             "surrogate2d": build_surrogate2d}
+             "surrogate2d": build_surrogate2d, "surrogate_infer": build_surrogate_infer}
 
 
 # ---------------------------------------------------------------------------
@@ -471,6 +572,8 @@ def run_batch(torch, wl, args):
     from pyftle_amd.tuning import best_batch_config
 
     cfg = best_batch_config("batch64") or BatchConfig()
+    if args.no_graph:
+        cfg.use_graph = 0
     plan = BatchPlan(wl["windows"], method="rk4", time_mode="linear", cfg=cfg)
     info = plan.info()
     log(f"batch: {info}")
@@ -538,6 +641,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="abc128", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batch64: plain launches instead of the captured hipGraph (for per-dispatch counters)")
     ap.add_argument("--surrogate-tol", type=float, default=1e-3,
                     help="surrogate2d: ensemble spread above which a seed falls back to RK4")
     ap.add_argument("--surrogate-epochs", type=int, default=0, help="surrogate2d: training epochs (0 = 80)")
@@ -582,6 +687,8 @@ def main():
         return run_batch(torch, wl, args)
     if wl["name"] == "surrogate2d":
         return run_surrogate(torch, wl, args)
+    if wl["name"] == "surrogate_infer":
+        return run_surrogate_infer(torch, wl, args)
     cfg = best_config(shape_class(wl["dim"], wl["n_seeds"], wl["time_mode"]))
     if args.groups_per_block > 0:
         cfg.groups_per_block = args.groups_per_block
