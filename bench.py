@@ -318,6 +318,164 @@ def run_surrogate(torch, wl, args):
     plan.close()
 
 
+def _abc_unsteady_field_on_device(torch, n, t_first, n_slots, dt_snap):
+    """Unsteady ABC flow, A(t) = sqrt(3) + 0.5 sin(pi t / 5), B = sqrt(2), C = 1, on a
+    periodic n^3 grid over [0, 2 pi)^3 (no endpoint), FP32 PAIR snapshots at
+    t_first + k dt_snap, k < n_slots."""
+    from pyftle_amd import _native as N
+    from pyftle_amd.field import DeviceField, GridGeometry
+
+    L = 2.0 * math.pi
+    geom = GridGeometry.uniform((n, n, n), (0.0,) * 3, (L / n,) * 3, periodic=(True,) * 3)
+    field = DeviceField(geom, dtype="float32", layout=N.LAYOUT_PAIR, ring_slots=n_slots, n_snapshots=n_slots,
+                        t0=t_first, dt_snap=dt_snap)
+    ax = torch.arange(n, dtype=torch.float64, device="cuda") * (L / n)
+    Z, Y, X = torch.meshgrid(ax, ax, ax, indexing="ij")  # flat ix + n*(iy + n*iz)
+    B, C = math.sqrt(2.0), 1.0
+    sx, cx, sy, cy, sz, cz = X.sin(), X.cos(), Y.sin(), Y.cos(), Z.sin(), Z.cos()
+    del X, Y, Z
+    for k in range(n_slots):
+        a = math.sqrt(3.0) + 0.5 * math.sin(math.pi * (t_first + k * dt_snap) / 5.0)
+        field.pack(k, torch.stack([a * sz + C * cy, B * sx + a * cz, C * sy + B * cx]).reshape(3, -1))
+    torch.cuda.synchronize()
+    return field
+
+
+def _abc_cpu_workload(torch, wl, seeds, delta=1e-3):
+    """The 3D surrogate's flow as the reference CPU path sees it: the snapshot at the
+    first window start, frozen, on the host, and the 6-neighbour groups of the seeds."""
+    import numpy as np
+
+    n = wl["config"]["grid"][0]
+    L = 2.0 * math.pi
+    ax = np.arange(n) * (L / n)
+    Z, Y, X = np.meshgrid(ax, ax, ax, indexing="ij")
+    a = math.sqrt(3.0) + 0.5 * math.sin(math.pi * wl["t0s"][0] / 5.0)
+    B, C = math.sqrt(2.0), 1.0
+    vel = np.stack([a * np.sin(Z) + C * np.cos(Y), B * np.sin(X) + a * np.cos(Z),
+                    C * np.sin(Y) + B * np.cos(X)]).reshape(3, -1)
+    blocks = []
+    for c in range(3):
+        e = torch.zeros(3, dtype=torch.float64, device=seeds.device)
+        e[c] = delta
+        blocks += [seeds - e, seeds + e] if c == 0 else [seeds + e, seeds - e]
+    return dict(name="surrogate3d", pos=torch.cat(blocks, 0), n_seeds=seeds.shape[0], dim=3, h=wl["h"],
+                steps=int(round(wl["T"] / wl["h"])), host=dict(vel=vel, n=n, sp=L / n))
+
+
+def build_surrogate3d(torch, n=192, ns=256, t0s=(0.0, 2.5, 5.0, 7.5), members=4, epochs=40, n_traj=4_000_000):
+    """Unsteady ABC on a periodic 192^3 grid: 64 snapshots per period 10 (dt = 10/64),
+    stored from t = -1.25 to 16.25 so that training windows start in [-1.25, 11.25)
+    and the 4 inference windows (T = 5, RK4 h = 0.01) lie inside."""
+    from pyftle_amd import surrogate_ftle as SF
+
+    dts = 10.0 / 64.0
+    field = _abc_unsteady_field_on_device(torch, n, -1.25, 113, dts)
+    return dict(name="surrogate3d", field=field, ns=ns, t0s=list(t0s), members=members, epochs=epochs,
+                n_traj=n_traj, T=5.0, h=0.01, dim=3, SF=SF,
+                config={"workload": "surrogate3d", "flow": "ABC, A(t)=sqrt3+0.5 sin(pi t/5), B=sqrt2, C=1",
+                        "grid": [n, n, n], "periodic": True, "snapshots_per_period": 64,
+                        "stored_time": [-1.25, 16.25], "seeds": [ns, ns, ns], "neighbors": 6, "delta": 1e-3,
+                        "t0_values": list(t0s), "T": 5.0, "rk4_h": 0.01,
+                        "mlp": "64 Fourier freqs (sigma 0.5) on (x,y,z,t0), 6x512 SiLU -> 3",
+                        "members": members, "epochs": epochs, "trajectories": n_traj,
+                        "judge": "RK4 on a 1% held-out seed subset (rng seed 1)",
+                        "dtype": "bf16 MFMA, fp32 accumulate"})
+
+
+def run_surrogate3d(torch, wl, args):
+    """3D surrogate FTLE fields against plain GPU RK4: accuracy on the held-out 1% of
+    the seeds, time for all 256^3 seeds of a window (both with HIP events)."""
+    from pyftle_amd import surrogate as S
+
+    SF = wl["SF"]
+    field = wl["field"]
+    L = 2.0 * math.pi
+    t = time.perf_counter()
+    X, D = SF.make_training_data(field, wl["n_traj"], wl["T"], wl["h"], (-1.25, 11.25), (0.0,) * 3, (L,) * 3,
+                                 n_bins=400)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t
+    spec = S.MLPSpec(d_in=4, d_out=3, n_freq=64, width=512, depth=6, sigma=0.5, scale=(0.5, 0.5, 0.5, 0.1),
+                     out_scale=2.0)
+    t = time.perf_counter()
+    ens = SF.fit_ensemble(X, D, spec, members=wl["members"], epochs=wl["epochs"], batch=16384, lr=2e-3, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t
+    del X, D
+    ns = wl["ns"]
+    dev = torch.device("cuda")
+    sax = (torch.arange(ns, device=dev, dtype=torch.float64) + 0.5) * (L / ns)
+    cz, cy, cx = torch.meshgrid(sax, sax, sax, indexing="ij")
+    seeds = torch.stack([cx.reshape(-1), cy.reshape(-1), cz.reshape(-1)], 1).contiguous()
+    del cx, cy, cz
+    g = torch.Generator(device=dev).manual_seed(1)
+    judge_idx = torch.randperm(seeds.shape[0], generator=g, device=dev)[: seeds.shape[0] // 100].sort().values
+    judge_seeds = seeds[judge_idx].contiguous()
+    sf = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], tol=args.surrogate_tol, chunk=1 << 19)
+
+    def timed(fn, reps=1):
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            out = fn()
+        b.record()
+        torch.cuda.synchronize()
+        return out, a.elapsed_time(b) / reps
+
+    n_seeds = seeds.shape[0]
+    _, rk4_ms = timed(lambda: sf.rk4_ftle(seeds, wl["t0s"][0]))
+    log(f"plain RK4, all {n_seeds} seeds, one window: {rk4_ms:.1f} ms")
+    per_t0 = []
+    for t0 in wl["t0s"]:
+        ref = sf.rk4_ftle(judge_seeds, t0)
+        row = {"t0": t0}
+        for jm in (wl["members"], 1):
+            sf.jacobian_members = jm
+            (f_all, spread), ms = timed(lambda: sf.predict(seeds, t0))
+            f_j = f_all[judge_idx]
+            row[f"jac{jm}"] = {"ms": ms, "rmse": SF.relative_rmse(f_j, ref)}
+            if jm == wl["members"]:
+                sp_j = spread[judge_idx]
+                for tol in (1e-2, 3e-2, 1e-1, 3e-1):
+                    bad = sp_j > tol
+                    comp = torch.where(bad, ref, f_j)
+                    row[f"tol{tol:g}"] = {"fallback": bad.float().mean().item(), "rmse": SF.relative_rmse(comp, ref)}
+        log(f"t0 {t0}: " + ", ".join(f"{k} {v}" for k, v in row.items() if k != "t0"))
+        per_t0.append(row)
+    sf.jacobian_members = wl["members"]
+    res, sur_ms = timed(lambda: sf.ftle(seeds, wl["t0s"][0]))
+    fb = res.fallback.float().mean().item()
+    del res
+    mean = lambda key, sub: sum(r[key][sub] for r in per_t0) / len(per_t0)  # noqa: E731
+    fpq = S.mlp_flops_per_query(spec)
+    jm4 = mean(f"jac{wl['members']}", "ms")
+    rows4 = n_seeds * wl["members"] * 4
+    result = {
+        "metric": METRIC, "value": n_seeds / (sur_ms * 1e-3), "unit": "seeds/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": sur_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16", "data": "synthetic", "config": dict(wl["config"], fallback_tol=args.surrogate_tol),
+        "surrogate": {
+            "with_fallback": {"tol": args.surrogate_tol, "ms": sur_ms, "fallback_fraction": fb},
+            "ftle_rmse_over_range_mean_jac_all": mean(f"jac{wl['members']}", "rmse"),
+            "ftle_rmse_over_range_mean_jac_1": mean("jac1", "rmse"),
+            "ms_surrogate_only_jac_all": jm4, "ms_surrogate_only_jac_1": mean("jac1", "ms"),
+            "per_t0": per_t0,
+            "mfma": {"achieved": rows4 * fpq / (jm4 * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": rows4 * fpq / (jm4 * 1e-3) / 1e12 / 2500.0, "flops_per_query": fpq,
+                     "kernel": "gemm_bf16_kernel (layer by layer, width 512)"},
+            "train_s": t_train, "data_s": t_data},
+        "rk4": {"ms_all_seeds_one_window": rk4_ms, "seeds_per_s": n_seeds / (rk4_ms * 1e-3)},
+        "speedup_vs_gpu_rk4": rk4_ms / sur_ms,
+        "surrogate_only_speedup_vs_gpu_rk4": {"jac_all": rk4_ms / jm4, "jac_1": rk4_ms / mean("jac1", "ms")},
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(_abc_cpu_workload(torch, wl, seeds), budget_s=15.0),
+    }
+    print(json.dumps(result), flush=True)
+
+
 def build_surrogate_infer(torch, nsx=2048, nsy=1024, n_t0=32, members=4):
     """Inference half of surrogate2d: the same 67.1M seed-windows through four
     random-init members of the same architecture (no training), for profiling the
@@ -420,7 +578,8 @@ def run_surrogate_infer(torch, wl, args):
 
 
 WORKLOADS = {"abc128": build_abc128, "gyre512": build_gyre, "batch64": build_batch64,
-             "surrogate2d": build_surrogate2d, "surrogate_infer": build_surrogate_infer}
+             "surrogate2d": build_surrogate2d, "surrogate_infer": build_surrogate_infer,
+             "surrogate3d": build_surrogate3d}
 
 
 # ---------------------------------------------------------------------------
@@ -506,7 +665,7 @@ def cpu_baseline(wl, budget_s=20.0):
     n = wl["n_seeds"]
     nb = 2 * wl["dim"]
     h = wl["host"]
-    if wl["name"] == "abc128":
+    if wl["dim"] == 3:
         nn = h["n"]
         comps = [h["vel"][c].reshape(nn, nn, nn).transpose(2, 1, 0) for c in range(3)]
         path = ReferenceGridPath(comps, [0.0] * 3, [h["sp"]] * 3)
@@ -689,6 +848,8 @@ def main():
         return run_surrogate(torch, wl, args)
     if wl["name"] == "surrogate_infer":
         return run_surrogate_infer(torch, wl, args)
+    if wl["name"] == "surrogate3d":
+        return run_surrogate3d(torch, wl, args)
     cfg = best_config(shape_class(wl["dim"], wl["n_seeds"], wl["time_mode"]))
     if args.groups_per_block > 0:
         cfg.groups_per_block = args.groups_per_block

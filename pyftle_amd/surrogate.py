@@ -216,14 +216,19 @@ class FlowMapMLP:
 
     # -- workspace ---------------------------------------------------------
     def _workspace(self, Mp: int, blocks: int):
-        """Activation buffers, shared by every network of the same shape (one size resident)."""
+        """Activation buffers, shared by every network of the same shape (the two most
+        recently used sizes stay resident: an ensemble alternates tangent and primal-only
+        members)."""
         s = self.spec
         key = (s.d_in, s.d_out, s.n_freq, s.width, s.depth, Mp, blocks, str(self.params.device))
-        ws = _WORKSPACE.get(key)
+        ws = _WORKSPACE.pop(key, None)
+        if ws is not None:
+            _WORKSPACE[key] = ws  # most recent last
         if ws is None:
             dev = self.params.device
             R = Mp * blocks
-            _WORKSPACE.clear()
+            while len(_WORKSPACE) >= 2:
+                _WORKSPACE.pop(next(iter(_WORKSPACE)))
             ws = {
                 "x": torch.zeros(Mp, s.d_in, dtype=torch.float32, device=dev),
                 "F": torch.empty(R, 2 * s.n_freq, dtype=torch.bfloat16, device=dev),

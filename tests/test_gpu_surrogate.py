@@ -111,3 +111,41 @@ def test_jvp_ftle_on_linear_flow():
     ftle = S.ftle_from_displacement_jacobian(Jd, T).cpu().numpy()
     print(f"[gpu] linear-flow FTLE mean {ftle.mean():.4f} (exact {lam})", flush=True)
     assert abs(np.median(ftle) - lam) < 0.02
+
+
+def test_surrogate_3d_periodic_pipeline():
+    """Unsteady periodic ABC (3 tangents, 3x3 Jacobians, layer-by-layer kernels at width
+    128): the fitted ensemble's FTLE tracks RK4, and forced fallback returns RK4 exactly."""
+    import math as m_
+
+    from pyftle_amd import _native as N
+    from pyftle_amd import surrogate as S
+    from pyftle_amd import surrogate_ftle as SF
+    from pyftle_amd.field import DeviceField, GridGeometry
+
+    n, L, dts = 32, 2 * m_.pi, 0.25
+    geom = GridGeometry.uniform((n, n, n), (0.0,) * 3, (L / n,) * 3, periodic=(True,) * 3)
+    f = DeviceField(geom, dtype="float32", layout=N.LAYOUT_PAIR, ring_slots=13, n_snapshots=13, t0=0.0,
+                    dt_snap=dts)
+    ax = torch.arange(n, dtype=torch.float64, device="cuda") * (L / n)
+    Z, Y, X = torch.meshgrid(ax, ax, ax, indexing="ij")
+    for k in range(13):
+        a = m_.sqrt(3.0) + 0.5 * m_.sin(m_.pi * k * dts / 5.0)
+        f.pack(k, torch.stack([a * Z.sin() + Y.cos(), m_.sqrt(2.0) * X.sin() + a * Z.cos(),
+                               Y.sin() + m_.sqrt(2.0) * X.cos()]).reshape(3, -1))
+    T, h = 1.0, 0.02
+    X_, D_ = SF.make_training_data(f, 131072, T, h, (0.0, 2.0), (0.0,) * 3, (L,) * 3, n_bins=16)
+    spec = S.MLPSpec(d_in=4, d_out=3, n_freq=64, width=128, depth=3, sigma=0.5, scale=(0.5, 0.5, 0.5, 0.5))
+    ens = SF.fit_ensemble(X_, D_, spec, members=2, epochs=20, batch=8192, lr=3e-3)
+    g = torch.Generator(device="cuda").manual_seed(1)
+    seeds = torch.rand(4096, 3, dtype=torch.float64, device="cuda", generator=g) * L
+    sf = SF.SurrogateFTLE(ens, f, T, h, tol=1e9)
+    res = sf.ftle(seeds, 1.0)
+    ref = sf.rk4_ftle(seeds, 1.0)
+    assert bool(torch.isfinite(res.ftle).all())
+    err = SF.relative_rmse(res.ftle, ref)
+    print(f"[gpu] 3D surrogate FTLE rmse/range {err:.4f}", flush=True)
+    assert err < 0.15
+    sf.tol = -1.0  # every seed disagrees: the whole field comes from RK4
+    res = sf.ftle(seeds, 1.0)
+    assert bool(res.fallback.all()) and torch.equal(res.ftle, ref)
