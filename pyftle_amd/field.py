@@ -139,6 +139,45 @@ class GridGeometry:
                             per, strides, n)
 
 
+class HostStager:
+    """Pinned, double-buffered host -> device staging of snapshots.
+
+    ``stage(array)`` copies a host snapshot into pinned buffer ``b`` (a host
+    memcpy), enqueues ``b``'s device copy on a separate copy stream and makes
+    the caller's stream wait for it with an event, so the file reading and
+    memcpy of snapshot k+1 overlap the transfer and packing of snapshot k.
+    ``release(b)`` records, on the caller's stream, the event that marks buffer
+    ``b`` free again once the work that reads it has run.
+    """
+
+    def __init__(self, n_elems: int, dtype: torch.dtype, device: torch.device, buffers: int = 2):
+        self.host = [torch.empty(n_elems, dtype=dtype, pin_memory=True) for _ in range(buffers)]
+        self.dev = [torch.empty(n_elems, dtype=dtype, device=device) for _ in range(buffers)]
+        self.copy_stream = torch.cuda.Stream(device=device)  # default priority
+        self.free = [None] * buffers
+        self.next = 0
+
+    def stage(self, array: np.ndarray, stream: torch.cuda.Stream | None = None):
+        b = self.next
+        self.next = (b + 1) % len(self.host)
+        if self.free[b] is not None:
+            self.free[b].synchronize()  # the previous copy into dev[b] and its reader have run
+        n = array.size
+        np.copyto(self.host[b][:n].numpy().reshape(array.shape), array, casting="same_kind")
+        cur = stream if stream is not None else torch.cuda.current_stream()
+        with torch.cuda.stream(self.copy_stream):
+            self.dev[b][:n].copy_(self.host[b][:n], non_blocking=True)
+            ready = torch.cuda.Event()
+            ready.record(self.copy_stream)
+        cur.wait_event(ready)
+        return b, self.dev[b][:n]
+
+    def release(self, b: int, stream: torch.cuda.Stream | None = None) -> None:
+        ev = torch.cuda.Event()
+        ev.record(stream if stream is not None else torch.cuda.current_stream())
+        self.free[b] = ev
+
+
 class DeviceField:
     """Ring of snapshot slots resident in HBM plus its C descriptor."""
 
@@ -203,10 +242,28 @@ class DeviceField:
         return k % self.ring_slots
 
     def pack(self, k: int, velocities, strides=None, stream=None) -> None:
-        """Copy snapshot ``k`` (component-planar, reference layout) into its slot."""
+        """Copy snapshot ``k`` (component-planar, reference layout) into its slot.
+
+        Host arrays of float32/float64 go through pinned double-buffered staging
+        (``HostStager``); device tensors are packed directly.
+        """
+        if isinstance(velocities, np.ndarray) and velocities.dtype in (np.float32, np.float64) \
+                and strides is None:
+            tdt = torch.float64 if velocities.dtype == np.float64 else torch.float32
+            st = getattr(self, "_stager", None)
+            if st is None or st.host[0].numel() < velocities.size or st.host[0].dtype != tdt:
+                st = self._stager = HostStager(velocities.size, tdt, cuda_device())
+            b, src = st.stage(velocities, stream)
+            self._pack_device(k, src, None, stream)
+            st.release(b, stream)
+            return
         src = velocities if isinstance(velocities, torch.Tensor) else to_device(velocities)
         if not src.is_cuda:
             src = src.to(cuda_device())
+        self._pack_device(k, src, strides, stream)
+        self._keepalive = src  # the copy is asynchronous: keep the source alive
+
+    def _pack_device(self, k: int, src: torch.Tensor, strides, stream) -> None:
         if src.dtype not in (torch.float32, torch.float64):
             src = src.to(torch.float64)
         src = src.contiguous()
@@ -219,7 +276,6 @@ class DeviceField:
         N.call("pyftle_pack_snapshot", ctypes.byref(self._desc), int(k), ptr(src),
                dtype_code(src.dtype), arr, stream_handle(stream))
         self.resident[self.slot_of(k)] = k
-        self._keepalive = src  # the copy is asynchronous: keep the source alive
 
     def sample(self, points, time_mode=N.TIME_STEADY, snapshot=0, t=0.0):
         """Velocity at physical points (``HighPerfInterpolator.interpolate``)."""
