@@ -215,3 +215,40 @@ def test_euler_kat_linear_field():
     create_integrator("rk4", interp).integrate(0.1, parts)
     z = 0.01
     np.testing.assert_allclose(parts.positions, p0 * (1 + z + z**2 / 2 + z**3 / 6 + z**4 / 24), rtol=1e-14)
+
+
+@pytest.mark.parametrize("method", ["rk4", "euler"])
+def test_file_experiment_batched_equals_window_by_window(tmp_path, method):
+    """All windows of a file experiment in one persistent launch give the FTLE fields of
+    FTLESolver run window by window, bit for bit (forward and backward time)."""
+    from scipy.io import savemat
+
+    from pyftle_amd.experiment import FileFTLEManager, load_config
+
+    nx, ny = 64, 32
+    vels, pts = H.double_gyre_window(nx=nx, ny=ny, n_snap=12)
+    cx, cy = pts[0].reshape(ny, nx), pts[1].reshape(ny, nx)
+    savemat(tmp_path / "c.mat", {"coordinate_x": pts[0], "coordinate_y": pts[1]})
+    for k, v in enumerate(vels):
+        savemat(tmp_path / f"v{k:02d}.mat", {"velocity_x": v[0], "velocity_y": v[1]})
+    x0 = H.seeds_2d(30, 15)
+    n = x0.shape[0] // 4
+    for j in range(2):
+        blocks = {key: x0[i * n:(i + 1) * n] + 0.01 * j for i, key in enumerate(("left", "right", "top", "bottom"))}
+        savemat(tmp_path / f"p{j}.mat", blocks)
+    (tmp_path / "v.txt").write_text("\n".join(str(tmp_path / f"v{k:02d}.mat") for k in range(12)))
+    (tmp_path / "c.txt").write_text(str(tmp_path / "c.mat"))
+    (tmp_path / "p.txt").write_text("\n".join(str(tmp_path / f"p{j}.mat") for j in range(2)))
+    assert cx.shape == (ny, nx) and cy.shape == (ny, nx)
+    for dt in (0.25, -0.25):
+        cfg = load_config(None, experiment_name="t", list_velocity_files=str(tmp_path / "v.txt"),
+                          list_coordinate_files=str(tmp_path / "c.txt"),
+                          list_particle_files=str(tmp_path / "p.txt"), snapshot_timestep=dt,
+                          flow_map_period=1.0, integrator=method, interpolator="grid",
+                          flow_grid_shape=f"{nx},{ny}")
+        m = FileFTLEManager(cfg, write=False)
+        batched = m.run(batched=True)
+        single = m.run(batched=False)
+        assert len(batched) == len(single) == 12 - 5 + 1
+        for a, b in zip(batched, single):
+            np.testing.assert_array_equal(a, b)

@@ -183,3 +183,84 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_parse_tuple_rules():
+    from pyftle_amd.experiment import parse_tuple
+
+    assert parse_tuple("10,20,30") == (10, 20, 30)
+    assert parse_tuple("128, 128 , 256") == (128, 128, 256)
+    assert parse_tuple("10,20,1") == (10, 20)
+    for bad in ("10", "1,2,3,4", "0,5", "-1,4"):
+        with pytest.raises(ValueError):
+            parse_tuple(bad)
+
+
+def test_load_config_keys_and_required(tmp_path):
+    from pyftle_amd.experiment import load_config
+
+    cfg_file = tmp_path / "config.yaml"
+    cfg_file.write_text(
+        'experiment_name: "exp"\nlist_velocity_files: "v.txt"\nlist_coordinate_files: "c.txt"\n'
+        'list_particle_files: "p.txt"\nsnapshot_timestep: -0.1\nflow_map_period: 5\nintegrator: rk4\n'
+        'interpolator: grid\nnum_processes: 8\noutput_format: "mat"\nflow_grid_shape: 128, 128 , 256\n'
+        'particles_grid_shape: 64, 32, 1\nnew_key: 3\n')
+    c = load_config(str(cfg_file))
+    assert c.flow_grid_shape == (128, 128, 256) and c.particles_grid_shape == (64, 32)
+    assert c.snapshot_timestep == -0.1 and c.flow_map_period == 5.0 and c.num_processes == 8
+    assert c.extra == {"new_key": 3}
+    assert load_config(str(cfg_file), integrator="euler").integrator == "euler"
+    (tmp_path / "short.yaml").write_text("experiment_name: x\n")
+    with pytest.raises(ValueError, match="missing required"):
+        load_config(str(tmp_path / "short.yaml"))
+
+
+def _write_lists(tmp_path, n_snap, n_coord=1, n_part=1):
+    vs = [str(tmp_path / f"v{k:03d}.mat") for k in range(n_snap)]
+    cs = [str(tmp_path / f"c{k}.mat") for k in range(n_coord)]
+    ps = [str(tmp_path / f"p{k}.mat") for k in range(n_part)]
+    for name, items in (("v.txt", vs), ("c.txt", cs), ("p.txt", ps)):
+        (tmp_path / name).write_text("\n".join(items) + "\n")
+    return vs, cs, ps
+
+
+def test_file_windows_follow_the_reference_construction(tmp_path):
+    from pyftle_amd.experiment import FileFTLEManager, load_config
+
+    vs, cs, ps = _write_lists(tmp_path, 7, n_coord=2, n_part=3)
+    base = dict(experiment_name="e", list_velocity_files=str(tmp_path / "v.txt"),
+                list_coordinate_files=str(tmp_path / "c.txt"), list_particle_files=str(tmp_path / "p.txt"),
+                flow_map_period=0.75, integrator="rk4", interpolator="grid", flow_grid_shape="4,4")
+    m = FileFTLEManager(load_config(None, snapshot_timestep=0.25, **base), write=False)
+    bs = m.create_batches()
+    assert m.window_length() == 4 and len(bs) == 4
+    assert [b.snapshot_files for b in bs] == [vs[i:i + 4] for i in range(4)]
+    coords = (cs * 10)[:7 + 4 - 1]
+    assert [b.coordinate_files for b in bs] == [coords[i:i + 4] for i in range(4)]
+    assert [b.particle_file for b in bs] == [ps[i % 3] for i in range(4)]
+    back = FileFTLEManager(load_config(None, snapshot_timestep=-0.25, **base), write=False)
+    assert back.create_batches()[0].snapshot_files == vs[::-1][:4]
+
+
+def test_mat_writer_and_factory(tmp_path):
+    from scipy.io import loadmat
+
+    from pyftle_amd.file_writers import MatWriter, create_writer
+
+    w = create_writer("mat", str(tmp_path / "out"), (3, 2))
+    assert isinstance(w, MatWriter)
+    cen = np.arange(12.0).reshape(6, 2)
+    w.write("ftle_a", np.arange(6.0), cen)
+    d = loadmat(str(tmp_path / "out" / "ftle_a.mat"))
+    # 2D grids are stored as (nx, ny, 1), as the reference's writer does
+    np.testing.assert_array_equal(d["ftle"], np.arange(6.0).reshape(3, 2, 1))
+    np.testing.assert_array_equal(d["x"], cen[:, 0].reshape(3, 2, 1))
+    create_writer("mat", str(tmp_path / "flat")).write("f", np.arange(4.0), np.ones((4, 3)))
+    assert set(loadmat(str(tmp_path / "flat" / "f.mat"))) >= {"ftle", "x", "y", "z"}
+    with pytest.raises(ValueError, match="Unsupported output format"):
+        create_writer("csv", str(tmp_path))
+    try:
+        import pyvista  # noqa: F401
+    except ImportError:
+        with pytest.raises(RuntimeError, match="pyvista"):
+            create_writer("vtk", str(tmp_path))
