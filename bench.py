@@ -789,7 +789,17 @@ def run_batch(torch, wl, args):
                 del f64
         result["accuracy"] = {"batch_equals_single_window_launches": same,
                               "fp64_fields_checked": checked, "max_abs_err_over_range": worst}
-    result["cpu_baseline"] = None
+    cpu = None
+    if not args.no_cpu:
+        # the reference CPU path on the batch's most common window shape (512x256 gyre,
+        # T = 10), converted to the batch's seeds/s through its mean steps per seed
+        cpu = cpu_baseline(_gyre_cpu_workload(torch, {"config": {"grid": [512, 256]}, "h": 0.05, "T": 10.0}),
+                           budget_s=15.0)
+        if cpu is not None:
+            cpu["value"] = cpu["seed_steps_per_s"] * wl["n_seeds"] / wl["seed_steps"]
+            cpu["sample"] += (f"; batch seeds/s = seed-steps/s x {wl['n_seeds']} seeds / "
+                              f"{wl['seed_steps']} seed-steps of the batch")
+    result["cpu_baseline"] = cpu
     print(json.dumps(result), flush=True)
     plan.close()
 
