@@ -251,6 +251,14 @@ int pyftle_mlp_mse(const float *Y, const float *T, int64_t n, float scale, float
 int pyftle_mlp_adam(float *w, float *m, float *v, const float *g, int64_t n, float lr, float beta1,
                     float beta2, float eps, int32_t step, float weight_decay, void *w_bf16,
                     void *stream);
+/* Adam with a cosine learning-rate schedule evaluated on the device from *step (the
+ * number k of completed schedule steps; incremented by the call on the stream; bias
+ * corrections for Adam step adam_step0 + k + 1), so a training step holds no host-side
+ * scalars and can be captured in a graph. */
+int pyftle_mlp_adam_scheduled(float *w, float *m, float *v, const float *g, int64_t n, float beta1,
+                              float beta2, float eps, float weight_decay, float lr_max, float lr_min,
+                              int32_t total_steps, int32_t adam_step0, int32_t *step, void *w_bf16,
+                              void *stream);
 int pyftle_mlp_cast_bf16(const float *x, int64_t n, void *y, void *stream);
 
 /* Fused inference (Fourier features, every hidden layer and the output layer in one

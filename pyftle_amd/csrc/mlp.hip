@@ -298,6 +298,27 @@ __global__ void __launch_bounds__(256) reduce_splits_kernel(const float *part, i
   out[i] = s;
 }
 
+// Same sum with many partials: a block reduces 4 outputs, 64 threads per output each
+// summing every 64th partial in order, then a fixed-order tree in LDS (deterministic,
+// like the one-thread-per-output kernel).
+__global__ void __launch_bounds__(256) reduce_splits_wide_kernel(const float *part, int splits, int64_t n,
+                                                                 float *out) {
+  __shared__ float red[4][64];
+  const int l = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int64_t i = (int64_t)blockIdx.x * 4 + q;
+  float s = 0.0f;
+  if (i < n)
+    for (int k = l; k < splits; k += 64) s += part[(int64_t)k * n + i];
+  red[q][l] = s;
+  __syncthreads();
+#pragma unroll
+  for (int w = 32; w > 0; w >>= 1) {
+    if (l < w) red[q][l] += red[q][l + w];
+    __syncthreads();
+  }
+  if (l == 0 && i < n) out[i] = red[q][0];
+}
+
 // Partial column sums over a chunk of rows: part[c, n] = sum_{m in chunk c} X[m, n].
 __global__ void __launch_bounds__(256) colsum_kernel(const __bf16 *X, int64_t ld, int M, int N, int rows,
                                                      float *part) {
@@ -340,6 +361,33 @@ __global__ void __launch_bounds__(256) adam_kernel(float *w, float *m, float *v,
   w[i] = wi;
   if (wb) wb[i] = (__bf16)wi;
 }
+
+// Adam with the cosine learning-rate schedule evaluated on the device from the number
+// of completed steps k (*step): lr = lr_min + (lr_max - lr_min) (1 + cos(pi k / total)) / 2,
+// bias corrections for Adam step t0 + k + 1 (t0: steps taken before this schedule).  Nothing changes on the host between steps, so a
+// training step can be captured once in a graph and replayed.
+__global__ void __launch_bounds__(256) adam_scheduled_kernel(float *w, float *m, float *v, const float *g,
+                                                             int64_t n, float b1, float b2, float eps,
+                                                             float wd, float lr_max, float lr_min,
+                                                             int total, int t0, const int *step, __bf16 *wb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = *step;
+  const double c = cos(3.141592653589793 * (double)k / (double)(total > 0 ? total : 1));
+  const float lr = (float)((double)lr_min + 0.5 * ((double)lr_max - (double)lr_min) * (1.0 + c));
+  const float bc1 = 1.0f - powf(b1, (float)(t0 + k + 1));
+  const float bc2 = 1.0f - powf(b2, (float)(t0 + k + 1));
+  const float gi = g[i] + wd * w[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = (__bf16)wi;
+}
+
+__global__ void step_increment_kernel(int *step) { *step += 1; }
 
 __global__ void __launch_bounds__(256) cast_bf16_kernel(const float *x, int64_t n, __bf16 *y) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -445,7 +493,10 @@ int pyftle_mlp_linear_out_backward(const float *dY, const float *W, int32_t O, i
 int pyftle_mlp_reduce_splits(const float *part, int32_t splits, int64_t n, float *out, void *stream) {
   if (!part || !out || splits < 1 || n < 0) return fail("mlp_reduce_splits: bad arguments");
   if (n == 0) return PYFTLE_OK;
-  reduce_splits_kernel<<<nb256(n), 256, 0, as_stream(stream)>>>(part, splits, n, out);
+  if (splits >= 32 && n * splits <= (int64_t)1 << 26)
+    reduce_splits_wide_kernel<<<(unsigned)((n + 3) / 4), 256, 0, as_stream(stream)>>>(part, splits, n, out);
+  else
+    reduce_splits_kernel<<<nb256(n), 256, 0, as_stream(stream)>>>(part, splits, n, out);
   return check_launch("mlp_reduce_splits");
 }
 
@@ -476,6 +527,19 @@ int pyftle_mlp_adam(float *w, float *m, float *v, const float *g, int64_t n, flo
   adam_kernel<<<nb256(n), 256, 0, as_stream(stream)>>>(w, m, v, g, n, lr, beta1, beta2, eps, bc1, bc2,
                                                        weight_decay, (__bf16 *)w_bf16);
   return check_launch("mlp_adam");
+}
+
+int pyftle_mlp_adam_scheduled(float *w, float *m, float *v, const float *g, int64_t n, float beta1, float beta2,
+                              float eps, float weight_decay, float lr_max, float lr_min, int32_t total_steps,
+                              int32_t adam_step0, int32_t *step, void *w_bf16, void *stream) {
+  if (!w || !m || !v || !g || !step || n < 0 || total_steps < 1 || adam_step0 < 0)
+    return fail("mlp_adam_scheduled: bad arguments");
+  hipStream_t st = as_stream(stream);
+  if (n > 0)
+    adam_scheduled_kernel<<<nb256(n), 256, 0, st>>>(w, m, v, g, n, beta1, beta2, eps, weight_decay, lr_max,
+                                                    lr_min, total_steps, adam_step0, step, (__bf16 *)w_bf16);
+  step_increment_kernel<<<1, 1, 0, st>>>(step);
+  return check_launch("mlp_adam_scheduled");
 }
 
 int pyftle_mlp_cast_bf16(const float *x, int64_t n, void *y, void *stream) {

@@ -42,6 +42,8 @@ _SIGS = {
     "pyftle_mlp_adam": [_vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _i32, _f32, _vp, _vp],
     "pyftle_mlp_cast_bf16": [_vp, _i64, _vp, _vp],
     "pyftle_mlp_pack_fused": [_vp, _i32, _i32, _i32, _vp, _vp],
+    "pyftle_mlp_adam_scheduled": [_vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _f32, _f32, _i32, _i32, _vp,
+                                  _vp, _vp],
     "pyftle_mlp_fused_forward": [_vp, _i32, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _f32,
                                  _i32, _vp, _vp],
 }
@@ -291,7 +293,7 @@ class FlowMapMLP:
         s = self.spec
         ws = self._ws[(Mp, 1)]
         dev = dY.device
-        chunk = 256  # rows per partial sum of the narrow reductions
+        chunk = 32  # rows per partial sum of the narrow reductions (many partials: parallelism)
         chunks = (Mp + chunk - 1) // chunk
         g = self.grads
         o, shp, ob = self.out_off
@@ -303,7 +305,9 @@ class FlowMapMLP:
                _p(ws["Z"][s.depth - 1]), s.width, chunk, _p(dWp), _p(dbp), _p(dZ), stream_handle())
         N.call("pyftle_mlp_reduce_splits", _p(dWp), chunks, s.d_out * s.width, _off(g, o), stream_handle())
         N.call("pyftle_mlp_reduce_splits", _p(dbp), chunks, s.d_out, _off(g, ob), stream_handle())
-        splits = max(1, Mp // 4096)
+        # weight gradients are (out x in) GEMMs over K = Mp rows: split K so that the
+        # 2-4 output tiles still give a few hundred work-groups
+        splits = max(1, min(64, Mp // 256))
         while Mp % (splits * 64):
             splits -= 1
         F2 = 2 * s.n_freq
@@ -331,26 +335,66 @@ class FlowMapMLP:
 
 
 def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20, batch: int = 32768,
-          lr: float = 2e-3, lr_min: float = 2e-5, seed: int = 0, log=None):
-    """Adam with a cosine learning-rate schedule over minibatches; returns per-epoch losses."""
+          lr: float = 2e-3, lr_min: float = 2e-5, seed: int = 0, log=None, graph: bool = False):
+    """Adam with a cosine learning-rate schedule over minibatches; returns per-epoch losses.
+
+    Each epoch gathers a shuffled copy of the data once; a step takes the next ``batch``
+    rows at a device-side offset, runs forward, backward and the device-scheduled Adam
+    update (``pyftle_mlp_adam_scheduled``), so no launch argument changes between steps.
+    With ``graph`` the step is captured in a HIP graph after two eager steps and replayed;
+    eager and replayed steps run the same kernels on the same buffers (bit-identical
+    weights).  At width 256 and 16384 rows a step (~40 kernels, 0.56 ms) is GPU-bound, so
+    replay gains nothing there and eager is the default.
+    """
     n = X.shape[0]
-    steps_per_epoch = max(1, n // batch)
+    if n < batch:
+        raise ValueError(f"need at least one batch of {batch} rows, got {n}")
+    dev = X.device
+    steps_per_epoch = n // batch
     total = epochs * steps_per_epoch
-    gen = torch.Generator(device=X.device).manual_seed(seed)
+    gen = torch.Generator(device=dev).manual_seed(seed)
+    Xs, Ts = torch.empty_like(X), torch.empty_like(T)
+    xb = torch.empty(batch, X.shape[1], dtype=X.dtype, device=dev)
+    tb = torch.empty(batch, T.shape[1], dtype=T.dtype, device=dev)
+    base = torch.zeros(1, dtype=torch.int64, device=dev)
+    ar = torch.arange(batch, dtype=torch.int64, device=dev)
+    k_dev = torch.zeros(1, dtype=torch.int32, device=dev)
+    acc = torch.zeros(1, dtype=torch.float32, device=dev)
+    step0 = model.step
+
+    def body():
+        idx = base + ar
+        torch.index_select(Xs, 0, idx, out=xb)
+        torch.index_select(Ts, 0, idx, out=tb)
+        model.loss_and_backward(xb, tb, loss_out=acc)
+        N.call("pyftle_mlp_adam_scheduled", _p(model.params), _p(model.m), _p(model.v), _p(model.grads),
+               model.n_params, 0.9, 0.999, 1e-8, 0.0, float(lr), float(lr_min), total, step0, _p(k_dev),
+               _p(model.params_bf16), stream_handle())
+        base.add_(batch)
+
+    g = None
     losses = []
-    k = 0
+    done = 0
     for ep in range(epochs):
-        perm = torch.randperm(n, generator=gen, device=X.device)
-        acc = torch.zeros(1, dtype=torch.float32, device=X.device)
+        perm = torch.randperm(n, generator=gen, device=dev)
+        torch.index_select(X, 0, perm, out=Xs)
+        torch.index_select(T, 0, perm, out=Ts)
+        base.zero_()
+        acc.zero_()
         for i in range(steps_per_epoch):
-            idx = perm[i * batch:(i + 1) * batch]
-            model.loss_and_backward(X[idx].contiguous(), T[idx].contiguous(), loss_out=acc)
-            cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * k / max(1, total)))
-            model.adam_step(cur)
-            k += 1
+            if graph and g is None and done >= 2:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    body()
+            if g is not None:
+                g.replay()
+            else:
+                body()
+            done += 1
         losses.append(acc.item() / steps_per_epoch)
         if log:
             log(f"epoch {ep} loss {losses[-1]:.3e}")
+    model.step += total
     return losses
 
 
