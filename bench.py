@@ -200,7 +200,7 @@ def build_batch64(torch, h=0.05, n_fields=64):
                         "seed_steps_total": seed_steps})
 
 
-def build_surrogate2d(torch, nsx=2048, nsy=1024, n_t0=32, members=4, epochs=80, n_traj=4_000_000):
+def build_surrogate2d(torch, nsx=2048, nsy=1024, n_t0=32, members=4, epochs=200, n_traj=4_000_000):
     """Double-gyre flow-map surrogate: 512x256 snapshots over t in [0, 20] (the flow is
     10-periodic, windows start in [0, 10)), RK4 h=0.05 over T=10."""
     from pyftle_amd import surrogate_ftle as SF
@@ -363,7 +363,7 @@ def _abc_cpu_workload(torch, wl, seeds, delta=1e-3):
                 steps=int(round(wl["T"] / wl["h"])), host=dict(vel=vel, n=n, sp=L / n))
 
 
-def build_surrogate3d(torch, n=192, ns=256, t0s=(0.0, 2.5, 5.0, 7.5), members=4, epochs=40, n_traj=4_000_000):
+def build_surrogate3d(torch, n=192, ns=256, t0s=(0.0, 2.5, 5.0, 7.5), members=4, epochs=100, n_traj=4_000_000):
     """Unsteady ABC on a periodic 192^3 grid: 64 snapshots per period 10 (dt = 10/64),
     stored from t = -1.25 to 16.25 so that training windows start in [-1.25, 11.25)
     and the 4 inference windows (T = 5, RK4 h = 0.01) lie inside."""
@@ -814,7 +814,8 @@ def main():
                     help="batch64: plain launches instead of the captured hipGraph (for per-dispatch counters)")
     ap.add_argument("--surrogate-tol", type=float, default=1e-3,
                     help="surrogate2d: ensemble spread above which a seed falls back to RK4")
-    ap.add_argument("--surrogate-epochs", type=int, default=0, help="surrogate2d: training epochs (0 = 80)")
+    ap.add_argument("--surrogate-epochs", type=int, default=0,
+                    help="surrogate2d / surrogate3d: training epochs (0 = the workload's default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy check")
     ap.add_argument("--groups-per-block", type=int, default=0)
@@ -848,8 +849,8 @@ def main():
     _native.load()
     torch.cuda.set_device(0)
     log(f"building workload {args.workload}")
-    if args.workload == "surrogate2d" and args.surrogate_epochs > 0:
-        wl = build_surrogate2d(torch, epochs=args.surrogate_epochs)
+    if args.workload in ("surrogate2d", "surrogate3d") and args.surrogate_epochs > 0:
+        wl = WORKLOADS[args.workload](torch, epochs=args.surrogate_epochs)
     else:
         wl = WORKLOADS[args.workload](torch)
     if wl["name"] == "batch64":
