@@ -5,7 +5,10 @@ Drop-in names (pyFTLE 1.0.3 ``src/pyftle``):
 ``AnalyticalInterpolator``, ``create_integrator`` and the Euler/AB2/RK4
 integrators, ``compute_flow_map_jacobian_2x2/3x3``, ``compute_cauchy_green_*``,
 ``max_eigenvalue_*``, ``compute_ftle_*``, ``FTLESolver``, the ``BatchSource``
-implementations and ``AnalyticalSolver``.  Compute runs in hand-written HIP
+implementations, ``AnalyticalSolver``, the ``config.yaml`` keys (``load_config``,
+``FileFTLEManager``) and the ``.mat`` writer.  Additions: ``BatchPlan`` (many FTLE
+windows in one persistent launch) and the learned flow-map surrogate
+(``FlowMapMLP``, ``FlowMapEnsemble``, ``SurrogateFTLE``).  Compute runs in hand-written HIP
 kernels (``libpyftle_hip.so``); there is no CPU fallback.
 """
 
@@ -39,6 +42,14 @@ def __getattr__(name):
         "FileBatchSource": ".data_source",
         "AnalyticalBatchSource": ".data_source",
         "ArrayBatchSource": ".data_source",
+        "FileFTLEManager": ".experiment",
+        "load_config": ".experiment",
+        "create_writer": ".file_writers",
+        "BatchPlan": ".batch",
+        "Window": ".batch",
+        "FlowMapMLP": ".surrogate",
+        "FlowMapEnsemble": ".surrogate",
+        "SurrogateFTLE": ".surrogate_ftle",
     }
     if name in table:
         mod = importlib.import_module(table[name], __name__)
