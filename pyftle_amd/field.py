@@ -63,16 +63,73 @@ def ptr(t: torch.Tensor) -> ctypes.c_void_p:
     return ctypes.c_void_p(t.data_ptr())
 
 
+PINNED_MIN_BYTES = 1 << 20  # host arrays at least this large move through pinned buffers
+_PINNABLE = {np.dtype(np.float32): torch.float32, np.dtype(np.float64): torch.float64,
+             np.dtype(np.int32): torch.int32, np.dtype(np.int64): torch.int64}
+
+
+_COPY_THREADS = 4
+_copy_pool = None
+
+
+def host_copy(dst: np.ndarray, src: np.ndarray) -> None:
+    """``np.copyto(dst, src)``; large contiguous copies are split over a few
+    threads (numpy releases the GIL for plain copies, and one core's memcpy
+    falls well short of the host memory bandwidth)."""
+    global _copy_pool
+    if src.nbytes < (2 << 20) or dst.shape != src.shape or not (dst.flags.c_contiguous and
+                                                                 src.flags.c_contiguous):
+        np.copyto(dst, src, casting="same_kind")
+        return
+    if _copy_pool is None:
+        from concurrent.futures import ThreadPoolExecutor
+
+        _copy_pool = ThreadPoolExecutor(max_workers=_COPY_THREADS, thread_name_prefix="pyftle-copy")
+    d = dst.reshape(-1)
+    s_ = src.reshape(-1)
+    cuts = np.linspace(0, d.size, _COPY_THREADS + 1).astype(np.int64)
+    futs = [_copy_pool.submit(np.copyto, d[a:b], s_[a:b], "same_kind") for a, b in zip(cuts[:-1], cuts[1:])]
+    for f in futs:
+        f.result()
+
+
 def to_device(x, dtype: torch.dtype | None = None) -> torch.Tensor:
+    """Device copy of a tensor or host array.  Large host arrays are copied into a
+    pinned buffer (torch's caching host allocator, which keeps the block until the
+    transfer has run) and sent asynchronously; pageable transfers go through small
+    driver staging chunks at a fraction of the bus rate."""
     dev = cuda_device()
     if isinstance(x, torch.Tensor):
         t = x.to(device=dev, dtype=dtype if dtype is not None else x.dtype)
     else:
         arr = np.asarray(x)
-        t = torch.from_numpy(np.ascontiguousarray(arr)).to(device=dev)
+        tdt = _PINNABLE.get(arr.dtype)
+        if tdt is not None and arr.nbytes >= PINNED_MIN_BYTES:
+            host = torch.empty(arr.shape, dtype=tdt, pin_memory=True)
+            host_copy(host.numpy(), arr)
+            t = host.to(device=dev, non_blocking=True)
+        else:
+            t = torch.from_numpy(np.ascontiguousarray(arr)).to(device=dev)
         if dtype is not None:
             t = t.to(dtype)
     return t.contiguous()
+
+
+def to_host(t: torch.Tensor, out: np.ndarray | None = None) -> np.ndarray:
+    """Host copy of a device tensor; large ones come back through a pinned buffer
+    (one DMA at the bus rate instead of the driver's pageable staging chunks).
+    With ``out`` the values are written into that array (same shape) in place."""
+    if t.numel() * t.element_size() >= PINNED_MIN_BYTES:
+        host = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+        host.copy_(t, non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        h = host.numpy()
+    else:
+        h = t.cpu().numpy()
+    if out is None:
+        return h
+    host_copy(out, h.reshape(out.shape))
+    return out
 
 
 @dataclass
@@ -143,17 +200,18 @@ class HostStager:
     """Pinned, double-buffered host -> device staging of snapshots.
 
     ``stage(array)`` copies a host snapshot into pinned buffer ``b`` (a host
-    memcpy), enqueues ``b``'s device copy on a separate copy stream and makes
-    the caller's stream wait for it with an event, so the file reading and
-    memcpy of snapshot k+1 overlap the transfer and packing of snapshot k.
-    ``release(b)`` records, on the caller's stream, the event that marks buffer
-    ``b`` free again once the work that reads it has run.
+    memcpy) and enqueues ``b``'s device copy on the caller's stream, so the file
+    reading and memcpy of snapshot k+1 overlap the transfer and packing of
+    snapshot k.  ``release(b)`` records, on the caller's stream, the event that
+    marks buffer ``b`` free again once the work that reads it has run.  (A
+    separate copy stream, joined by an event per snapshot, measured slower: the
+    transfer is short next to the host memcpy, and each cross-stream wait costs
+    tens of microseconds.)
     """
 
     def __init__(self, n_elems: int, dtype: torch.dtype, device: torch.device, buffers: int = 2):
         self.host = [torch.empty(n_elems, dtype=dtype, pin_memory=True) for _ in range(buffers)]
         self.dev = [torch.empty(n_elems, dtype=dtype, device=device) for _ in range(buffers)]
-        self.copy_stream = torch.cuda.Stream(device=device)  # default priority
         self.free = [None] * buffers
         self.next = 0
 
@@ -163,13 +221,10 @@ class HostStager:
         if self.free[b] is not None:
             self.free[b].synchronize()  # the previous copy into dev[b] and its reader have run
         n = array.size
-        np.copyto(self.host[b][:n].numpy().reshape(array.shape), array, casting="same_kind")
+        host_copy(self.host[b][:n].numpy().reshape(array.shape), array)
         cur = stream if stream is not None else torch.cuda.current_stream()
-        with torch.cuda.stream(self.copy_stream):
+        with torch.cuda.stream(cur):
             self.dev[b][:n].copy_(self.host[b][:n], non_blocking=True)
-            ready = torch.cuda.Event()
-            ready.record(self.copy_stream)
-        cur.wait_event(ready)
         return b, self.dev[b][:n]
 
     def release(self, b: int, stream: torch.cuda.Stream | None = None) -> None:

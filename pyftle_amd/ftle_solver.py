@@ -23,11 +23,10 @@ from __future__ import annotations
 
 from typing import Optional
 
-import numpy as np
 import torch
 
 from . import window as W
-from .field import to_device
+from .field import to_device, to_host
 from .integrate import Integrator
 from .interpolate import AnalyticalInterpolator, HighPerfInterpolator
 
@@ -105,10 +104,10 @@ class FTLESolver:
             self.progress_queue.put((id_, "done"))
 
         if on_host:
-            np.copyto(pos, dev.cpu().numpy())
+            to_host(dev, out=pos)
         else:
             pos.copy_(dev)
-        ftle_field = ftle.cpu().numpy() if on_host else ftle
+        ftle_field = to_host(ftle) if on_host else ftle
 
         if self.output_writer is not None:
             self.output_writer.write(f"ftle_{id_}", ftle_field, self.particles.initial_centroid)

@@ -146,8 +146,12 @@ def test_ftle_solver_array_source_matches_oracle():
     np.testing.assert_array_equal(parts.positions, exp_pos)
 
 
-def test_ftle_solver_chunked_ring_matches_single_launch():
-    """A ring smaller than the window (several launches) gives the same field."""
+@pytest.mark.parametrize("slots", [1, 2, 5])
+@pytest.mark.parametrize("on_device", [False, True])
+def test_ftle_solver_chunked_ring_matches_single_launch(slots, on_device):
+    """A ring smaller than the window (several launches, packing on a second stream
+    overlapped with the previous launch) gives the oracle's field; device snapshots
+    written on the compute stream just before the window are seen whole."""
     from pyftle_amd import NeighboringParticles
     from pyftle_amd.data_source import ArrayBatchSource
     from pyftle_amd.ftle_solver import FTLESolver
@@ -160,9 +164,15 @@ def test_ftle_solver_chunked_ring_matches_single_launch():
         fields = H.grid_fields((64, 32), vels, pts)
         exp_ftle, exp_pos = O.run_window_reference([f.interpolate for f in fields], x0, 0.1, method)
         small = HighPerfInterpolator((64, 32), precision="float64",
-                                     max_ring_bytes=5 * 64 * 32 * 2 * 8)
+                                     max_ring_bytes=slots * 64 * 32 * 2 * 8)
+        src_vels = vels
+        if on_device:
+            # filled by a kernel on the current stream, not yet complete when run() starts
+            src_vels = [torch.zeros(v.shape, dtype=torch.float64, device="cuda") for v in vels]
+            for d, v in zip(src_vels, vels):
+                d.add_(torch.as_tensor(v).to("cuda", non_blocking=True))
         parts = NeighboringParticles(x0.copy())
-        ftle = FTLESolver(ArrayBatchSource(vels, pts, parts, 0.1), create_integrator(method, small)).run()
+        ftle = FTLESolver(ArrayBatchSource(src_vels, pts, parts, 0.1), create_integrator(method, small)).run()
         np.testing.assert_allclose(ftle, exp_ftle, rtol=1e-12, atol=1e-14)
         np.testing.assert_array_equal(parts.positions, exp_pos)
 
