@@ -175,7 +175,10 @@ int pyftle_ftle_from_groups(const void *positions, const void *initial_positions
  * snapshot rings, seed counts, windows and step sizes may all differ.
  * pyftle_batch_create allocates the device work list and, with use_graph,
  * captures counter reset + persistent kernel into a hipGraph; launches then
- * replay it.  The fields' buffers must outlive the batch. */
+ * replay it.  The fields' buffers must outlive the batch.  Launches of one
+ * batch share its work counter: they must not overlap on different streams.
+ * pyftle_batch_destroy waits for the device (a replay may still run) and
+ * frees the batch. */
 typedef struct pyftle_batch_field {
   pyftle_field field;
   const void *positions_in; /* block-ordered (2*dim*n_groups, dim) of pyftle_batch_opts.pos_dtype */

@@ -88,7 +88,13 @@ class Window:
 
 
 class BatchPlan:
-    """A captured batch: ``run()`` recomputes every window's FTLE in one launch."""
+    """A captured batch: ``run()`` recomputes every window's FTLE in one launch.
+
+    The device pointers of every window (positions, outputs, snapshot rings) are
+    bound at construction and replayed by each ``run()``; the plan keeps the
+    windows alive.  Runs of one plan share its work counter, so two runs of the
+    same plan must not overlap on different streams.  ``close()`` waits for the
+    device before freeing the plan."""
 
     def __init__(self, windows: list[Window], method="rk4", time_mode="linear",
                  cfg: BatchConfig | None = None):
@@ -101,11 +107,21 @@ class BatchPlan:
         arr = (BatchField * len(windows))()
         for i, w in enumerate(windows):
             dim = w.field.geom.dim
-            if w.positions.dtype != pdt or not w.positions.is_cuda or not w.positions.is_contiguous():
+            p = w.positions
+            if p.dtype != pdt or not p.is_cuda or not p.is_contiguous():
                 raise ValueError("batch positions must be contiguous device tensors of one dtype")
-            n = w.positions.shape[0] // (2 * dim)
+            if p.dim() != 2 or p.shape[1] != dim or p.shape[0] % (2 * dim) != 0:
+                raise ValueError(f"window {i}: positions must have shape (2*{dim}*N, {dim}), got {tuple(p.shape)}")
+            n = p.shape[0] // (2 * dim)
+            po = w.positions_out
+            if po is not None and (po.shape != p.shape or po.dtype != p.dtype or not po.is_cuda
+                                   or not po.is_contiguous()):
+                raise ValueError(f"window {i}: positions_out must be a contiguous device tensor like positions")
             if w.ftle is None:
-                w.ftle = torch.empty(n, dtype=torch.float64, device=w.positions.device)
+                w.ftle = torch.empty(n, dtype=torch.float64, device=p.device)
+            elif (w.ftle.dtype != torch.float64 or w.ftle.numel() != n or not w.ftle.is_cuda
+                  or not w.ftle.is_contiguous()):
+                raise ValueError(f"window {i}: ftle must be a contiguous float64 device tensor of {n} values")
             bf = arr[i]
             bf.field = w.field.desc
             bf.positions_in = w.positions.data_ptr()

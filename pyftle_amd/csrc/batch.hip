@@ -252,7 +252,11 @@ int pyftle_batch_info(const pyftle_batch *b, int64_t *n_items, int32_t *grid_blo
 }
 
 int pyftle_batch_destroy(pyftle_batch *b) {
+  if (!b) return PYFTLE_OK;
+  // a replay of this plan may still be running: its graph, table and counter are freed below
+  hipError_t e = hipDeviceSynchronize();
   release(b);
+  if (e != hipSuccess) return fail(std::string("batch_destroy: ") + hipGetErrorString(e), PYFTLE_EHIP);
   return PYFTLE_OK;
 }
 

@@ -28,7 +28,6 @@ import os
 from dataclasses import dataclass, field
 from typing import List, Optional
 
-import numpy as np
 import torch
 
 from . import _native as N

@@ -21,11 +21,10 @@ import ctypes
 import math
 from dataclasses import dataclass
 
-import numpy as np
 import torch
 
 from . import _native as N
-from .field import ptr, stream_handle
+from .field import stream_handle
 
 EPI_F32, EPI_BIAS_SILU, EPI_DSILU, EPI_BIAS = 0, 1, 2, 3
 
@@ -198,10 +197,18 @@ class FlowMapMLP:
         if getattr(self, "_fused", None) is None or self._fused_step != self.step:
             self.pack_fused()
         Wp, bias, wout, bout = self._fused
+        if x.dim() != 2 or x.shape[1] != s.d_in or x.dtype != torch.float32 or x.device != self.params.device:
+            raise ValueError(f"x must be a float32 (M, {s.d_in}) tensor on {self.params.device}")
         x = x.contiguous()
         M = x.shape[0]
-        Y = out if out is not None else torch.empty(1 + n_tangents, M, s.d_out, dtype=torch.float32,
-                                                    device=x.device)
+        shape = (1 + n_tangents, M, s.d_out)
+        if out is None:
+            Y = torch.empty(shape, dtype=torch.float32, device=x.device)
+        elif tuple(out.shape) != shape or out.dtype != torch.float32 or not out.is_contiguous() \
+                or out.device != x.device:
+            raise ValueError(f"out must be a contiguous float32 tensor of shape {shape} on {x.device}")
+        else:
+            Y = out
         N.call("pyftle_mlp_fused_forward", _p(x), s.d_in, M, _p(self.scale), _p(self.B), s.n_freq, _p(Wp),
                _p(bias), s.depth, s.width, _p(wout), _p(bout), s.d_out, float(s.out_scale), int(n_tangents),
                _p(Y), stream_handle())
@@ -455,4 +462,4 @@ def mlp_flops_per_query(spec: MLPSpec) -> int:
 
 
 __all__ = ["MLPSpec", "FlowMapMLP", "FlowMapEnsemble", "train", "ftle_from_displacement_jacobian",
-           "gemm", "pad128", "mlp_flops_per_query", "np"]
+           "gemm", "pad128", "mlp_flops_per_query"]
