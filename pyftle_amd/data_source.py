@@ -1,8 +1,8 @@
 """Window feeds (data_source.py:29-285): the ``BatchSource`` protocol.
 
-* ``FileBatchSource``: one ``.mat`` velocity file per step, coordinates read on
-  step 0 and re-read only when the coordinate files differ
-  (data_source.py:86-211);
+* ``FileBatchSource``: one ``.mat`` velocity file per step (read ahead by a
+  ``Prefetcher``), coordinates read on step 0 and re-read only when the
+  coordinate files differ (data_source.py:86-211);
 * ``AnalyticalBatchSource``: closed-form velocity, no per-step update
   (data_source.py:214-285);
 * ``ArrayBatchSource`` (new): snapshots already in memory (host arrays or
@@ -14,7 +14,7 @@ from __future__ import annotations
 from pathlib import Path
 from typing import Callable, List, Protocol, Sequence, Tuple
 
-from .file_readers import read_coordinate, read_seed_particles_coordinates, read_velocity
+from .file_readers import Prefetcher, read_coordinate, read_seed_particles_coordinates, read_velocity
 from .particles import NeighboringParticles
 
 
@@ -46,6 +46,7 @@ class FileBatchSource:
         self._n = len(snapshot_files)
         self._id = f"{Path(self.snapshot_files[0]).stem}"
         self.reuse_coordinates = len(set(coordinate_files)) == 1
+        self._velocities = Prefetcher(snapshot_files, read_velocity)  # reads ahead on I/O threads
 
     @property
     def id(self) -> str:
@@ -63,7 +64,7 @@ class FileBatchSource:
         return read_seed_particles_coordinates(self.particle_file)
 
     def get_data_for_step(self, step_index: int) -> Tuple:
-        velocities = read_velocity(self.snapshot_files[step_index])
+        velocities = self._velocities.get(step_index)
         coordinates = None
         if step_index == 0 or not self.reuse_coordinates:
             coordinates = read_coordinate(self.coordinate_files[step_index])

@@ -33,8 +33,8 @@ import torch
 from . import _native as N
 from .batch import BatchConfig, BatchPlan, Window
 from .data_source import FileBatchSource
-from .field import DeviceField, GridGeometry, to_device
-from .file_readers import read_coordinate, read_seed_particles_coordinates, read_velocity
+from .field import DeviceField, GridGeometry, to_device, to_host
+from .file_readers import Prefetcher, read_coordinate, read_seed_particles_coordinates, read_velocity
 from .ftle_solver import FTLESolver
 from .integrate import create_integrator
 from .interpolate import create_interpolator
@@ -152,9 +152,12 @@ class FileFTLEManager:
             batched = self._can_batch(batches)
         results = self._run_batched(batches) if batched else self._run_sequential(batches)
         if self.writer is not None:
+            centroids = {}
             for b, r in zip(batches, results):
-                cen = read_seed_particles_coordinates(b.particle_file).initial_centroid
-                self.writer.write(f"ftle_{b.id}", r, cen)
+                key = os.path.abspath(b.particle_file)
+                if key not in centroids:
+                    centroids[key] = read_seed_particles_coordinates(b.particle_file).initial_centroid
+                self.writer.write(f"ftle_{b.id}", r, centroids[key])
             return None
         return results
 
@@ -176,19 +179,28 @@ class FileFTLEManager:
         geom = GridGeometry.from_reference(shape, read_coordinate(self.coordinate_files[0]))
         n, p = len(self.snapshot_files), self.window_length()
         field = DeviceField(geom, dtype="float32", layout=N.LAYOUT_SINGLE, ring_slots=n, n_snapshots=n)
-        for k, path in enumerate(self.snapshot_files):  # each file read once, staged through pinned buffers
-            field.pack(k, read_velocity(path))
+        # each file read once (ahead, on I/O threads) and staged through pinned buffers
+        for k, vel in enumerate(Prefetcher(self.snapshot_files, read_velocity, depth=8)):
+            field.pack(k, vel)
         method = {0: "euler", 1: "ab2", 2: "rk4"}[self.integrator.method]
         windows = []
+        seeds = {}  # particle file -> device positions (the kernel only reads them)
         for i, b in enumerate(batches):
-            pos = to_device(read_seed_particles_coordinates(b.particle_file).positions, torch.float64)
-            windows.append(Window(field=field, positions=pos, n_steps=p, h=self.timestep,
+            key = os.path.abspath(b.particle_file)
+            if key not in seeds:
+                seeds[key] = to_device(read_seed_particles_coordinates(b.particle_file).positions, torch.float64)
+            windows.append(Window(field=field, positions=seeds[key], n_steps=p, h=self.timestep,
                                   map_period=(p - 1) * abs(self.timestep), first_snapshot=i))
         plan = BatchPlan(windows, method=method, time_mode="frozen", cfg=BatchConfig())
         res = plan.run()
-        torch.cuda.synchronize()
+        sizes = [r.numel() for r in res]
+        flat = to_host(torch.cat(res))  # one transfer for every window's field
         plan.close()
-        return [r.cpu().numpy() for r in res]
+        out, o = [], 0
+        for n_ in sizes:
+            out.append(flat[o:o + n_])
+            o += n_
+        return out
 
 
 __all__ = ["ExperimentConfig", "load_config", "parse_tuple", "get_files_list", "FileFTLEManager"]

@@ -136,6 +136,33 @@ def test_file_batch_source_reuses_coordinates(tmp_path):
     assert src.num_steps == 3 and src.timestep == 0.1 and src.id == "v0"
     assert src.get_data_for_step(0)[1] is not None
     assert src.get_data_for_step(2)[1] is None
+    # read-ahead hands out exactly what read_velocity returns, in any order, repeatedly
+    from pyftle_amd.file_readers import read_velocity
+
+    for k in (1, 0, 2, 2, 1):
+        np.testing.assert_array_equal(src.get_data_for_step(k)[0],
+                                      read_velocity(str(tmp_path / f"v{k}.mat")))
+
+
+def test_prefetcher_order_and_errors(tmp_path):
+    from scipy.io import savemat
+
+    from pyftle_amd.file_readers import Prefetcher, read_velocity
+
+    paths = []
+    for k in range(7):
+        savemat(tmp_path / f"s{k}.mat", {"velocity_x": np.full((3, 2), k), "velocity_y": -np.arange(6.0)})
+        paths.append(str(tmp_path / f"s{k}.mat"))
+    got = list(Prefetcher(paths, read_velocity, depth=3))
+    assert len(got) == 7
+    for k, g in enumerate(got):
+        np.testing.assert_array_equal(g, read_velocity(paths[k]))
+    pf = Prefetcher(paths + [str(tmp_path / "missing.mat")], read_velocity, depth=8)
+    np.testing.assert_array_equal(pf.get(0)[0], np.zeros(6))
+    with pytest.raises(FileNotFoundError):
+        pf.get(7)
+    with pytest.raises(IndexError):
+        pf.get(8)
 
 
 def test_analytical_solver_windows():
