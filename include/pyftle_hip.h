@@ -19,6 +19,9 @@
  *   ftle.py:99-163          max_eigenvalue_2x2/3x3             -> pyftle_max_eigenvalue
  *   ftle.py:166-233         compute_ftle_2x2/3x3               -> pyftle_ftle_from_jacobian
  *   cauchy_green.py + ftle.py (fused)                          -> pyftle_ftle_from_groups
+ *   interpolate.py:546-762  GridInterpolator (SciPy RegularGridInterpolator, methods
+ *                           linear / nearest / cubic) -> pyftle_field.interp modes,
+ *                           pyftle_pack_spline (cubic coefficients)
  */
 #ifndef PYFTLE_HIP_H
 #define PYFTLE_HIP_H
@@ -57,6 +60,14 @@ extern "C" {
 #define PYFTLE_COMPUTE_MIXED 1  /* FP64 positions, cell index and weights; FP32 corner blend */
 #define PYFTLE_COMPUTE_F64 2    /* FP64 positions and FP64 blend of the FP32 records */
 
+/* ---- spatial interpolation of a field (pyftle_field.interp) ---- */
+#define PYFTLE_INTERP_GINTERP 0     /* GridInterp.cpp: multilinear, zero outside, upper edge excluded */
+#define PYFTLE_INTERP_RGI_LINEAR 1  /* RegularGridInterpolator(method="linear", bounds_error=False,
+                                       fill_value=0): upper edge included, NaN -> NaN */
+#define PYFTLE_INTERP_RGI_NEAREST 2 /* method="nearest": nearest node, ties to the lower node */
+#define PYFTLE_INTERP_RGI_CUBIC 3   /* method="cubic": tensor-product not-a-knot cubic spline; the
+                                       slot holds B-spline coefficients (pyftle_pack_spline) */
+
 /* ---- work-to-lane order for grouped kernels ---- */
 #define PYFTLE_ORDER_SEED_MAJOR 0     /* the nb particles of a seed sit in adjacent lanes */
 #define PYFTLE_ORDER_NEIGHBOR_MAJOR 1 /* a wave holds one neighbour direction of consecutive seeds */
@@ -69,6 +80,11 @@ extern "C" {
  * by pyftle_pack_snapshot.  Snapshot (or pair) k lives in slot k % ring_slots.
  * FP32 slots are limited to 2 GiB; rows and xy-slices to < 2^24 records.
  * Physical -> index mapping is (p - origin)/spacing (grid_interp.py:78-79).
+ * The RGI interp modes follow interpolate.py:546-762 instead: axes are
+ * numpy.linspace(origin, upper, n) (node k = k*spacing + origin, the last node
+ * is `upper`), no periodic axes, SINGLE layout, STEADY/FROZEN time; the CUBIC
+ * mode stores coefficient c[ix][iy][iz] (ix in -1..n) at record
+ * (ix+1) + (n[0]+2)*((iy+1) + (n[1]+2)*(iz+1)).
  */
 typedef struct pyftle_field {
   int32_t dim;          /* 2 or 3 */
@@ -84,6 +100,9 @@ typedef struct pyftle_field {
   double t0;            /* time of snapshot 0 (LINEAR mode) */
   double dt_snap;       /* time between snapshots (LINEAR mode, non-zero) */
   void *data;           /* device pointer to slot 0 */
+  int32_t interp;       /* PYFTLE_INTERP_* (0 = GridInterp.cpp semantics) */
+  int32_t reserved;     /* 0 */
+  double upper[3];      /* grid maximum per axis (last linspace node), RGI modes */
 } pyftle_field;
 
 typedef struct pyftle_advect_opts {
@@ -122,6 +141,17 @@ int pyftle_interp3d_vec(const double *v, int32_t X, int32_t Y, int32_t Z, const 
  * level 1 of pair slot k-1 (and level 1 of slot k when k is the last snapshot). */
 int pyftle_pack_snapshot(const pyftle_field *field, int32_t snapshot, const void *src,
                          int32_t src_dtype, const int64_t src_strides[4], void *stream);
+
+/* ---- cubic spline ingest: GridInterpolator(method="cubic").update (interpolate.py:546-762) ----
+ * Fits the tensor-product not-a-knot cubic spline of one snapshot (what SciPy's
+ * RegularGridInterpolator method="cubic" builds, solved exactly instead of by its
+ * default iterative solver) and stores its uniform B-spline coefficients in slot
+ * `snapshot` of a PYFTLE_INTERP_RGI_CUBIC field.  Source strides as for
+ * pyftle_pack_snapshot.  `work` is device scratch of at least
+ * pyftle_spline_work_bytes(field) bytes (FP64 node copy and two sweep buffers). */
+int64_t pyftle_spline_work_bytes(const pyftle_field *field);
+int pyftle_pack_spline(const pyftle_field *field, int32_t snapshot, const void *src,
+                       int32_t src_dtype, const int64_t src_strides[4], void *work, void *stream);
 
 /* ---- physical-space sampling: HighPerfInterpolator.interpolate (interpolate.py:475-543)
  * points (n,dim) of `pts_dtype`, out (n,dim) of the same dtype; velocity at time

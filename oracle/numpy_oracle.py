@@ -18,6 +18,11 @@ Pins (checked in ``tests/test_oracle.py``):
   ``numpy.linalg.eigvals`` and closed forms (identity -> 0, lambda_max = 4).
 * window driver: closed-form saddle ``u = l x, v = -l y`` and the window
   arithmetic of ``ftle_solver.py:99-160``.
+* ``RGIField``: the reference's ``GridInterpolator`` calls SciPy's
+  ``RegularGridInterpolator`` (SciPy 1.15.3 here, the version the reference
+  pins); the oracle makes the same calls.  Pinned by closed forms: multilinear
+  fields exact for ``linear``, tensor cubics exact for ``cubic`` with a direct
+  solver, grid nodes returned exactly by ``nearest``.
 """
 
 from __future__ import annotations
@@ -425,3 +430,40 @@ def run_window_linear(velocity_at, positions, t0, h, n_steps, integrator="rk4"):
             raise ValueError(integrator)
     T = n_steps * abs(h)
     return ftle_from_positions(x, x0, T), x
+
+
+# ---------------------------------------------------------------------------
+# GridInterpolator: interpolate.py:546-762 (SciPy RegularGridInterpolator)
+# ---------------------------------------------------------------------------
+class RGIField:
+    """One snapshot as the reference's ``GridInterpolator`` sees it after an update
+    with points: each component reshaped to ``grid_shape`` in C order
+    (interpolate.py:626-660), axes ``linspace(points[a].min(), points[a].max(), n)``
+    (interpolate.py:620-625), ``RegularGridInterpolator(..., method, bounds_error=False,
+    fill_value=0.0)`` per component; ``interpolate`` fills an ``(M, dim)`` array
+    (interpolate.py:680-700).  ``solver``: ``None`` = SciPy's default for spline
+    methods (an iterative solver), or e.g. ``scipy.sparse.linalg.spsolve`` for the
+    exact spline.
+    """
+
+    def __init__(self, grid_shape, velocities, points, method="linear", solver=None):
+        from scipy.interpolate import RegularGridInterpolator
+
+        grid_shape = tuple(int(n) for n in grid_shape)
+        pts = np.asarray(points, dtype=np.float64)
+        vel = np.asarray(velocities, dtype=np.float64)
+        self.dim = len(grid_shape)
+        self.grid = tuple(np.linspace(pts[a].min(), pts[a].max(), grid_shape[a]) for a in range(self.dim))
+        kw = {}
+        if solver is not None:
+            kw["solver"] = solver
+        self.comps = [RegularGridInterpolator(self.grid, vel[c].reshape(grid_shape), method=method,
+                                              bounds_error=False, fill_value=0.0, **kw)
+                      for c in range(self.dim)]
+
+    def interpolate(self, new_points):
+        new_points = np.asarray(new_points, dtype=np.float64)
+        out = np.empty_like(new_points)
+        for c in range(self.dim):
+            out[:, c] = self.comps[c](new_points)
+        return out

@@ -19,6 +19,7 @@ EULER, AB2, RK4 = 0, 1, 2
 TIME_STEADY, TIME_FROZEN, TIME_LINEAR = 0, 1, 2
 ORDER_SEED_MAJOR, ORDER_NEIGHBOR_MAJOR = 0, 1
 COMPUTE_NATIVE, COMPUTE_MIXED, COMPUTE_F64 = 0, 1, 2
+INTERP_GINTERP, INTERP_RGI_LINEAR, INTERP_RGI_NEAREST, INTERP_RGI_CUBIC = 0, 1, 2, 3
 OK, EINVAL, EHIP = 0, -1, -2
 
 
@@ -39,6 +40,9 @@ class Field(ctypes.Structure):
         ("t0", ctypes.c_double),
         ("dt_snap", ctypes.c_double),
         ("data", ctypes.c_void_p),
+        ("interp", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("upper", ctypes.c_double * 3),
     ]
 
 
@@ -75,6 +79,8 @@ SIGNATURES = {
     "pyftle_interp2d_vec": [_vp, _i32, _i32, _vp, _i64, _vp, _vp],
     "pyftle_interp3d_vec": [_vp, _i32, _i32, _i32, _vp, _i64, _vp, _vp],
     "pyftle_pack_snapshot": [_FP, _i32, _vp, _i32, ctypes.POINTER(ctypes.c_int64), _vp],
+    "pyftle_spline_work_bytes": [_FP],
+    "pyftle_pack_spline": [_FP, _i32, _vp, _i32, ctypes.POINTER(ctypes.c_int64), _vp, _vp],
     "pyftle_field_sample": [_FP, _i32, _i32, _f64, _vp, _i32, _i64, _vp, _vp],
     "pyftle_advect": [_FP, _OP, _vp, _i32, _i64, _vp, _vp],
     "pyftle_advect_ftle": [_FP, _OP, _vp, _i32, _i64, _vp, _f64, _vp, _vp],
@@ -105,6 +111,7 @@ def load(path: str | None = None):
         fn.argtypes = argtypes
         fn.restype = ctypes.c_int
     lib.pyftle_last_error.restype = ctypes.c_char_p
+    lib.pyftle_spline_work_bytes.restype = ctypes.c_int64
     if path is None:
         _lib = lib
     return lib

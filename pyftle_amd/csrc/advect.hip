@@ -34,6 +34,8 @@ static int check_opts(const pyftle_field *field, const pyftle_advect_opts *o) {
     return fail("advect: unknown compute precision");
   if (field->dtype == PYFTLE_F64 && o->compute != PYFTLE_COMPUTE_NATIVE)
     return fail("advect: an FP64 field always computes in FP64 (compute must be NATIVE)");
+  if (field->interp != PYFTLE_INTERP_GINTERP && o->time_mode == PYFTLE_TIME_LINEAR)
+    return fail("advect: RegularGridInterpolator modes run STEADY or FROZEN time");
   return PYFTLE_OK;
 }
 
@@ -44,7 +46,8 @@ static int prec_of(const pyftle_advect_opts *o) {
 }
 
 static void dispatch(const pyftle_field *field, const AdvectArgs &a, const LaunchSpec &sp) {
-  if (field->dim == 2) launch_advect_2d(a, sp);
+  if (field->interp != PYFTLE_INTERP_GINTERP) launch_advect_rgi(field->dim, field->interp, a, sp);
+  else if (field->dim == 2) launch_advect_2d(a, sp);
   else launch_advect_3d(a, sp);
 }
 

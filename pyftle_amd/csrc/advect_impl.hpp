@@ -21,6 +21,7 @@
 #include <type_traits>
 
 #include "common.hpp"
+#include "rgi.hpp"
 
 namespace pyftle {
 
@@ -137,6 +138,19 @@ __device__ __forceinline__ void sample(const FieldDev &f, const char *slot, doub
   }
 }
 
+// GridInterp (INTERP 0) or a RegularGridInterpolator mode (FP64 positions and blend).
+template <int DIM, bool PAIR, bool STORE64, int INTERP, typename Pos, typename Blend>
+__device__ __forceinline__ void sample_any(const FieldDev &f, const char *slot, double w, const Pos *x,
+                                           Blend *v) {
+  if constexpr (INTERP == PYFTLE_INTERP_GINTERP) {
+    sample<DIM, PAIR, STORE64>(f, slot, w, x, v);
+  } else {
+    static_assert(std::is_same<Pos, double>::value && std::is_same<Blend, double>::value,
+                  "RegularGridInterpolator modes run in FP64");
+    sample_rgi<DIM, INTERP, typename std::conditional<STORE64, double, float>::type>(f, slot, x, v);
+  }
+}
+
 // FTLE epilogue: LDS exchange of the final and initial positions of every
 // particle of the work-group, then lane sl < G forms the Jacobian of seed sl
 // (cauchy_green.py:70-74,140-151) and its FTLE in FP64.
@@ -173,7 +187,8 @@ __device__ __forceinline__ void ftle_epilogue(const AdvectArgs &a, int64_t lb, i
   }
 }
 
-template <int DIM, int METHOD, int TMODE, bool PAIR, bool STORE64, int PREC, bool FTLE>
+template <int DIM, int METHOD, int TMODE, bool PAIR, bool STORE64, int PREC, bool FTLE,
+          int INTERP = PYFTLE_INTERP_GINTERP>
 __global__ void __launch_bounds__(512) advect_kernel(AdvectArgs a) {
   using Pos = typename PrecTraits<PREC, STORE64>::Pos;
   using Blend = typename PrecTraits<PREC, STORE64>::Blend;
@@ -243,19 +258,19 @@ __global__ void __launch_bounds__(512) advect_kernel(AdvectArgs a) {
     Blend k1[DIM];
 #pragma unroll
     for (int c = 0; c < DIM; ++c) xs[c] = X[c].hi();
-    sample<DIM, PAIR, STORE64>(f, s0, w0, xs, k1);
+    sample_any<DIM, PAIR, STORE64, INTERP>(f, s0, w0, xs, k1);
     if (METHOD == PYFTLE_RK4) {
       Blend k2[DIM], k3[DIM], k4[DIM];
       const Pos hh = (Pos)(0.5 * a.h);
 #pragma unroll
       for (int c = 0; c < DIM; ++c) xs[c] = stage_pos<Pos>(X[c].hi(), hh, (Pos)k1[c]);
-      sample<DIM, PAIR, STORE64>(f, s1, w1, xs, k2);
+      sample_any<DIM, PAIR, STORE64, INTERP>(f, s1, w1, xs, k2);
 #pragma unroll
       for (int c = 0; c < DIM; ++c) xs[c] = stage_pos<Pos>(X[c].hi(), hh, (Pos)k2[c]);
-      sample<DIM, PAIR, STORE64>(f, s1, w1, xs, k3);
+      sample_any<DIM, PAIR, STORE64, INTERP>(f, s1, w1, xs, k3);
 #pragma unroll
       for (int c = 0; c < DIM; ++c) xs[c] = stage_pos<Pos>(X[c].hi(), h, (Pos)k3[c]);
-      sample<DIM, PAIR, STORE64>(f, s2, w2, xs, k4);
+      sample_any<DIM, PAIR, STORE64, INTERP>(f, s2, w2, xs, k4);
       const Pos h6 = (Pos)(a.h / 6.0);
 #pragma unroll
       for (int c = 0; c < DIM; ++c)
@@ -651,8 +666,9 @@ void launch_advect(const AdvectArgs &a, const LaunchSpec &sp) {
   else launch_time<DIM, PYFTLE_AB2>(a, sp);
 }
 
-// defined in advect_2d.hip / advect_3d.hip
+// defined in advect_2d.hip / advect_3d.hip / advect_rgi.hip
 void launch_advect_2d(const AdvectArgs &a, const LaunchSpec &sp);
 void launch_advect_3d(const AdvectArgs &a, const LaunchSpec &sp);
+void launch_advect_rgi(int dim, int interp, const AdvectArgs &a, const LaunchSpec &sp);
 
 }  // namespace pyftle

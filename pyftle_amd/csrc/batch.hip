@@ -132,6 +132,8 @@ int pyftle_batch_create(const pyftle_batch_field *fields, int32_t n_fields,
     if (bf.field.dim != dim || bf.field.layout != layout)
       return fail("batch_create: all fields must share dim and record layout");
     if (bf.field.dtype != PYFTLE_F32) return fail("batch_create: batches run FP32 fields");
+    if (bf.field.interp != PYFTLE_INTERP_GINTERP)
+      return fail("batch_create: batches run GridInterp-mode fields (interp 0)");
     if (opts->time_mode == PYFTLE_TIME_LINEAR &&
         (layout != PYFTLE_LAYOUT_PAIR || !(bf.field.dt_snap != 0.0)))
       return fail("batch_create: LINEAR time needs PAIR fields with non-zero dt_snap");

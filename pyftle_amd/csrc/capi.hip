@@ -37,12 +37,24 @@ int validate_field(const pyftle_field *f) {
   if (f->dtype != PYFTLE_F32 && f->dtype != PYFTLE_F64) return fail("field.dtype must be F32 or F64");
   if (f->layout != PYFTLE_LAYOUT_SINGLE && f->layout != PYFTLE_LAYOUT_PAIR)
     return fail("field.layout must be SINGLE or PAIR");
+  if (f->interp < PYFTLE_INTERP_GINTERP || f->interp > PYFTLE_INTERP_RGI_CUBIC)
+    return fail("field.interp must be one of PYFTLE_INTERP_*");
+  if (f->interp != PYFTLE_INTERP_GINTERP) {
+    if (f->layout != PYFTLE_LAYOUT_SINGLE) return fail("RegularGridInterpolator modes need the SINGLE layout");
+    for (int a = 0; a < f->dim; ++a) {
+      if (f->periodic[a]) return fail("RegularGridInterpolator modes have no periodic axes");
+      if (!isfinite(f->upper[a]) || !(f->spacing[a] > 0.0) || !(f->upper[a] > f->origin[a]))
+        return fail("RegularGridInterpolator modes need ascending axes: origin < upper, spacing > 0");
+      if (f->interp == PYFTLE_INTERP_RGI_CUBIC && f->n[a] < 4)
+        return fail("method cubic requires at least 4 points per dimension");
+    }
+  }
   if (f->ring_slots < 1) return fail("field.ring_slots must be >= 1");
   if (f->n_snapshots < 1) return fail("field.n_snapshots must be >= 1");
   if (!f->data) return fail("field.data is NULL");
   int64_t nodes = padded_nodes(f);
   if (nodes > (int64_t)INT32_MAX / 2) return fail("field grid too large (> 2^30 nodes)");
-  if (f->n[0] + 1 >= (1 << 24) || (int64_t)(f->n[0] + 1) * (f->n[1] + 1) >= (1 << 24))
+  if (axis_records(f, 0) >= (1 << 24) || (int64_t)axis_records(f, 0) * axis_records(f, 1) >= (1 << 24))
     return fail("field rows / slices must hold fewer than 2^24 nodes");
   int64_t need = nodes * record_bytes(f->dim, f->layout, f->dtype);
   if (f->slot_stride < need) return fail("field.slot_stride smaller than one slot of records");
@@ -78,10 +90,12 @@ int make_field_dev(const pyftle_field *f, FieldDev *d) {
     d->per_n[a] = d->periodic[a] ? d->n[a] : 0;
     d->lim[a] = d->periodic[a] ? d->n[a] : d->n[a] - 1;
   }
-  d->px = d->n[0] + d->periodic[0];
-  d->pxy = d->px * (d->n[1] + d->periodic[1]);
+  d->px = axis_records(f, 0);
+  d->pxy = d->px * (f->dim >= 2 ? axis_records(f, 1) : 1);
   d->t0 = f->t0;
   d->dts = f->dt_snap;
+  d->interp = f->interp;
+  for (int a = 0; a < 3; ++a) d->upper[a] = a < f->dim ? f->upper[a] : 0.0;
   return PYFTLE_OK;
 }
 

@@ -47,15 +47,24 @@ struct FieldDev {
   float n_f[3];
   float inv_n_f[3];
   double t0, dts;
+  int interp;       // PYFTLE_INTERP_*
+  double upper[3];  // last grid node per axis (RGI modes)
 };
 
 int make_field_dev(const pyftle_field *f, FieldDev *out);
 int validate_field(const pyftle_field *f);
 
-// nodes of one slot including the ghost node of every periodic axis
+// records per axis of one slot: n, plus the ghost node of a periodic axis, or
+// n + 2 spline coefficients (-1..n) in the cubic mode
+inline int axis_records(const pyftle_field *f, int a) {
+  if (f->interp == PYFTLE_INTERP_RGI_CUBIC) return f->n[a] + 2;
+  return f->n[a] + (f->periodic[a] ? 1 : 0);
+}
+
+// records of one slot
 inline int64_t padded_nodes(const pyftle_field *f) {
   int64_t g = 1;
-  for (int a = 0; a < f->dim; ++a) g *= (int64_t)f->n[a] + (f->periodic[a] ? 1 : 0);
+  for (int a = 0; a < f->dim; ++a) g *= (int64_t)axis_records(f, a);
   return g;
 }
 

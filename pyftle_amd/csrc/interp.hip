@@ -3,6 +3,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "rgi.hpp"
 
 namespace pyftle {
 
@@ -112,7 +113,51 @@ __global__ void __launch_bounds__(256) sample_kernel(FieldDev f, const char *slo
   for (int a = 0; a < DIM; ++a) out[i * DIM + a] = (P)v[a];
 }
 
+// RegularGridInterpolator modes (rgi.hpp), one lane per point, FP64 arithmetic.
+template <int DIM, int INTERP, typename T, typename P>
+__global__ void __launch_bounds__(256) sample_rgi_kernel(FieldDev f, const char *slot,
+                                                         const P *__restrict__ pts, int64_t n,
+                                                         P *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x[DIM], v[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) x[a] = (double)pts[i * DIM + a];
+  sample_rgi<DIM, INTERP, T>(f, slot, x, v);
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) out[i * DIM + a] = (P)v[a];
+}
+
 static inline unsigned blocks_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+template <int DIM, int INTERP, typename T>
+static void launch_sample_rgi_t(const FieldDev &d, const char *slot, const void *pts, int pd, int64_t n,
+                                void *out, hipStream_t st) {
+  if (pd == PYFTLE_F64)
+    sample_rgi_kernel<DIM, INTERP, T, double><<<blocks_for(n, 256), 256, 0, st>>>(
+        d, slot, (const double *)pts, n, (double *)out);
+  else
+    sample_rgi_kernel<DIM, INTERP, T, float><<<blocks_for(n, 256), 256, 0, st>>>(
+        d, slot, (const float *)pts, n, (float *)out);
+}
+
+template <int DIM, int INTERP>
+static void launch_sample_rgi_i(const FieldDev &d, bool f64, const char *slot, const void *pts, int pd,
+                                int64_t n, void *out, hipStream_t st) {
+  if (f64) launch_sample_rgi_t<DIM, INTERP, double>(d, slot, pts, pd, n, out, st);
+  else launch_sample_rgi_t<DIM, INTERP, float>(d, slot, pts, pd, n, out, st);
+}
+
+template <int DIM>
+static void launch_sample_rgi(const FieldDev &d, bool f64, const char *slot, const void *pts, int pd,
+                              int64_t n, void *out, hipStream_t st) {
+  if (d.interp == PYFTLE_INTERP_RGI_LINEAR)
+    launch_sample_rgi_i<DIM, PYFTLE_INTERP_RGI_LINEAR>(d, f64, slot, pts, pd, n, out, st);
+  else if (d.interp == PYFTLE_INTERP_RGI_NEAREST)
+    launch_sample_rgi_i<DIM, PYFTLE_INTERP_RGI_NEAREST>(d, f64, slot, pts, pd, n, out, st);
+  else
+    launch_sample_rgi_i<DIM, PYFTLE_INTERP_RGI_CUBIC>(d, f64, slot, pts, pd, n, out, st);
+}
 
 template <int DIM, bool PAIR, typename Real>
 static void launch_sample(const FieldDev &d, const char *slot, double w, const void *pts, int pd,
@@ -157,6 +202,8 @@ int pyftle_pack_snapshot(const pyftle_field *field, int32_t snapshot, const void
   if (src_dtype != PYFTLE_F32 && src_dtype != PYFTLE_F64) return fail("pack_snapshot: bad src dtype");
   if (snapshot < 0 || snapshot >= field->n_snapshots)
     return fail("pack_snapshot: snapshot index out of range");
+  if (field->interp == PYFTLE_INTERP_RGI_CUBIC)
+    return fail("pack_snapshot: cubic-mode slots hold spline coefficients (use pyftle_pack_spline)");
   const int dim = field->dim;
   const int nz = dim == 3 ? field->n[2] : 1;
   const int per_level = dim == 2 ? 2 : 4;
@@ -252,6 +299,11 @@ int pyftle_field_sample(const pyftle_field *field, int32_t time_mode, int32_t sn
   const char *slot = reinterpret_cast<const char *>(field->data) + (int64_t)s * field->slot_stride;
   hipStream_t st = as_stream(stream);
   const bool f64 = field->dtype == PYFTLE_F64;
+  if (field->interp != PYFTLE_INTERP_GINTERP) {
+    if (field->dim == 2) launch_sample_rgi<2>(d, f64, slot, points, pts_dtype, n, out, st);
+    else launch_sample_rgi<3>(d, f64, slot, points, pts_dtype, n, out, st);
+    return check_launch("field_sample");
+  }
   if (field->dim == 2) {
     if (pair) {
       if (f64) launch_sample<2, true, double>(d, slot, w, points, pts_dtype, n, out, st);

@@ -120,8 +120,8 @@ class FileFTLEManager:
         self.flow_map_period = cfg.flow_map_period
         self.precision = precision
         interp_name = cfg.interpolator or "cubic"
-        self.interpolator = create_interpolator(interp_name, cfg.flow_grid_shape, precision=precision) \
-            if interp_name.lower() == "grid" else create_interpolator(interp_name, cfg.flow_grid_shape)
+        kw = {"precision": precision} if cfg.flow_grid_shape is not None else {}
+        self.interpolator = create_interpolator(interp_name, cfg.flow_grid_shape, **kw)
         self.integrator = create_integrator(cfg.integrator or "euler", self.interpolator)
         self.writer = None
         if write or (write is None and cfg.output_format):

@@ -143,6 +143,7 @@ class GridGeometry:
     periodic: tuple
     src_strides: tuple  # element strides (c, x, y, z) of the reference flat layout
     grid_shape: tuple   # the user's grid_shape
+    upper: tuple = ()   # last axis node (the RegularGridInterpolator modes)
 
     @staticmethod
     def from_reference(grid_shape, points, periodic=None) -> "GridGeometry":
@@ -176,6 +177,35 @@ class GridGeometry:
             strides = (ngrid, nz, ny * nz, 1)
         per = tuple(bool(p) for p in periodic) if periodic is not None else (False,) * dim
         return GridGeometry(dim, n, tuple(origin), tuple(spacing), per, strides, grid_shape)
+
+    @staticmethod
+    def from_rgi(grid_shape, points) -> "GridGeometry":
+        """Geometry of ``GridInterpolator._initialize_interpolator`` (interpolate.py:598-660):
+        axis ``a`` is ``numpy.linspace(points[a].min(), points[a].max(), n_a)`` and each
+        velocity component is reshaped to ``grid_shape`` in C order (x slowest)."""
+        grid_shape = tuple(int(s) for s in grid_shape)
+        pts = points.detach().cpu().numpy() if isinstance(points, torch.Tensor) else np.asarray(points)
+        pts = pts.astype(np.float64, copy=False)
+        if pts.ndim != 2:
+            raise ValueError("points must have shape (dim, Ngrid)")
+        dim, n_points = pts.shape
+        expected = int(np.prod(grid_shape))
+        if expected != n_points:
+            raise ValueError(f"grid_shape {grid_shape} implies {expected} points, but got {n_points}.")
+        if dim not in (2, 3) or len(grid_shape) != dim:
+            raise ValueError("Velocity field must have 2 or 3 components (u, v, [w]).")
+        origin, spacing, upper = [], [], []
+        for a in range(dim):
+            axis = np.linspace(pts[a].min(), pts[a].max(), grid_shape[a])
+            origin.append(float(axis[0]))
+            upper.append(float(axis[-1]))
+            spacing.append(float((axis[-1] - axis[0]) / (grid_shape[a] - 1)))
+        if dim == 2:
+            strides = (n_points, grid_shape[1], 1, 0)
+        else:
+            strides = (n_points, grid_shape[1] * grid_shape[2], grid_shape[2], 1)
+        return GridGeometry(dim, grid_shape, tuple(origin), tuple(spacing), (False,) * dim, strides,
+                            grid_shape, tuple(upper))
 
     @staticmethod
     def uniform(n, origin, spacing, periodic=None) -> "GridGeometry":
@@ -237,8 +267,10 @@ class DeviceField:
     """Ring of snapshot slots resident in HBM plus its C descriptor."""
 
     def __init__(self, geom: GridGeometry, *, dtype=torch.float32, layout=N.LAYOUT_SINGLE,
-                 ring_slots: int = 1, n_snapshots: int = 1, t0: float = 0.0, dt_snap: float = 1.0):
+                 ring_slots: int = 1, n_snapshots: int = 1, t0: float = 0.0, dt_snap: float = 1.0,
+                 interp: int = N.INTERP_GINTERP):
         self.geom = geom
+        self.interp = int(interp)
         self.dtype = precision_dtype(dtype)
         self.layout = layout
         self.ring_slots = int(ring_slots)
@@ -248,8 +280,12 @@ class DeviceField:
         dim = geom.dim
         comps = (2 if dim == 2 else 4) * (2 if layout == N.LAYOUT_PAIR else 1)
         esize = 4 if self.dtype == torch.float32 else 8
-        # a periodic axis stores one ghost node (a copy of node 0) at index n
-        nodes = int(np.prod([geom.n[a] + (1 if geom.periodic[a] else 0) for a in range(dim)]))
+        # a periodic axis stores one ghost node (a copy of node 0) at index n; the cubic
+        # mode stores spline coefficients -1..n per axis
+        if self.interp == N.INTERP_RGI_CUBIC:
+            nodes = int(np.prod([geom.n[a] + 2 for a in range(dim)]))
+        else:
+            nodes = int(np.prod([geom.n[a] + (1 if geom.periodic[a] else 0) for a in range(dim)]))
         self.record_bytes = comps * esize
         self.slot_stride = -(-nodes * self.record_bytes // ALIGN) * ALIGN
         total = self.slot_stride * self.ring_slots + ALIGN
@@ -281,6 +317,9 @@ class DeviceField:
         d.t0 = self.t0
         d.dt_snap = self.dt_snap
         d.data = self.data_ptr
+        d.interp = self.interp
+        for a in range(3):
+            d.upper[a] = g.upper[a] if a < g.dim and len(g.upper) > a else 0.0
         return d
 
     @property
@@ -328,8 +367,17 @@ class DeviceField:
         if any(s < 0 for s in st) or src.numel() < extent:
             raise ValueError(f"snapshot source has {src.numel()} elements; its strides need {extent}")
         arr = (ctypes.c_int64 * 4)(*[int(s) for s in st])
-        N.call("pyftle_pack_snapshot", ctypes.byref(self._desc), int(k), ptr(src),
-               dtype_code(src.dtype), arr, stream_handle(stream))
+        if self.interp == N.INTERP_RGI_CUBIC:
+            work = getattr(self, "_spline_work", None)
+            if work is None:
+                nbytes = int(N.load().pyftle_spline_work_bytes(ctypes.byref(self._desc)))
+                work = self._spline_work = torch.empty(max(nbytes, 16) // 8 + 1, dtype=torch.float64,
+                                                       device=self._buf.device)
+            N.call("pyftle_pack_spline", ctypes.byref(self._desc), int(k), ptr(src), dtype_code(src.dtype),
+                   arr, ptr(work), stream_handle(stream))
+        else:
+            N.call("pyftle_pack_snapshot", ctypes.byref(self._desc), int(k), ptr(src),
+                   dtype_code(src.dtype), arr, stream_handle(stream))
         self.resident[self.slot_of(k)] = k
 
     def sample(self, points, time_mode=N.TIME_STEADY, snapshot=0, t=0.0):

@@ -13,9 +13,12 @@ Differences from the reference, by design:
 * Compute precision is a constructor argument: ``"float32"`` (default, the
   fast path) or ``"float64"`` (the reference's arithmetic, operation for
   operation).
-* The SciPy scattered/Delaunay strategies (``cubic``, ``linear``,
-  ``nearest``, interpolate.py:114-294,546-762) have no device implementation;
-  the factory raises ``NotImplementedError`` for them.
+* ``cubic``, ``linear`` and ``nearest`` with a ``grid_shape`` are the SciPy
+  ``RegularGridInterpolator`` strategies (``GridInterpolator``,
+  interpolate.py:546-762), run by the same device kernels in their own
+  interpolation modes.  Without a grid shape they are SciPy's scattered-data
+  strategies (Delaunay / KD-tree, interpolate.py:114-294), which have no
+  device implementation; the factory raises ``NotImplementedError``.
 """
 
 from __future__ import annotations
@@ -166,6 +169,62 @@ class HighPerfInterpolator(Interpolator):
         return res
 
 
+class GridInterpolator(HighPerfInterpolator):
+    """SciPy ``RegularGridInterpolator`` strategies on the device (interpolate.py:546-762).
+
+    ``method`` is ``"linear"`` (or ``"slinear"``, the same piecewise-linear
+    function), ``"nearest"`` or ``"cubic"``; as in the reference each velocity
+    component is reshaped to ``grid_shape`` in C order on the axes
+    ``numpy.linspace(points[a].min(), points[a].max(), n_a)``, points outside the
+    grid get 0 (``bounds_error=False, fill_value=0``), the upper edge belongs to
+    the grid, NaN positions give NaN.  ``"cubic"`` fits the tensor-product
+    not-a-knot cubic spline of every snapshot exactly (SciPy's default iterative
+    solver reaches it to ~1e-5); ``"quintic"`` is not provided.
+
+    Deliberate differences, both for updates without new points: the reference
+    re-reads later 2D snapshots in Fortran order although the first one was read
+    in C order, and its cubic splines stay fitted to the first snapshot
+    (``RegularGridInterpolator.values`` is replaced without a refit).  Here
+    every snapshot is read in C order and fitted.
+    """
+
+    VALID_METHODS = {"linear", "nearest", "slinear", "cubic", "quintic"}
+    _MODES = {"linear": N.INTERP_RGI_LINEAR, "slinear": N.INTERP_RGI_LINEAR,
+              "nearest": N.INTERP_RGI_NEAREST, "cubic": N.INTERP_RGI_CUBIC}
+
+    def __init__(self, grid_shape, method: str = "linear", precision="float32", ring_slots: int = 1,
+                 max_ring_bytes: int = 8 << 30):
+        if method not in self.VALID_METHODS:
+            raise ValueError(f"Invalid method '{method}'. Must be one of {self.VALID_METHODS}")
+        if method == "quintic":
+            raise NotImplementedError("GridInterpolator: quintic splines have no device implementation")
+        super().__init__(grid_shape, precision=precision, ring_slots=ring_slots, max_ring_bytes=max_ring_bytes)
+        self.method = method
+        self.mode = self._MODES[method]
+        if self.mode == N.INTERP_RGI_CUBIC and min(self.grid_shape) < 4:
+            i = int(np.argmin(self.grid_shape))
+            raise ValueError(f"There are {self.grid_shape[i]} points in dimension {i}, but method cubic "
+                             "requires at least  4 points per dimension.")
+
+    def slot_bytes(self) -> int:
+        comps = 2 if self.dim == 2 else 4
+        esize = 4 if self.dtype == torch.float32 else 8
+        ext = 2 if self.mode == N.INTERP_RGI_CUBIC else 0
+        return int(np.prod([n + ext for n in self.grid_shape])) * comps * esize
+
+    def _allocate(self) -> None:
+        self.field = DeviceField(self.geom, dtype=self.dtype, layout=N.LAYOUT_SINGLE,
+                                 ring_slots=self._ring, n_snapshots=1 << 30, interp=self.mode)
+
+    def _initialize_interpolator(self) -> None:
+        if self.velocities is None or self.points is None:
+            raise ValueError("Velocities and points must be set before initialization.")
+        geom = GridGeometry.from_rgi(self.grid_shape, self.points)
+        if self.field is None or geom != self.geom:
+            self.geom = geom
+            self._allocate()
+
+
 class AnalyticalInterpolator(Interpolator):
     """interpolate.py:765-829: ``velocity_fn(time, points)`` with time fixed at 0.
 
@@ -237,7 +296,9 @@ def create_interpolator(interpolation_type: str, grid_shape=None, velocity_fn=No
         if grid_shape is None:
             raise ValueError("the 'grid' interpolator needs grid_shape")
         return HighPerfInterpolator(grid_shape, **kwargs)
+    if grid_shape is not None:  # interpolate.py:895-896: RegularGridInterpolator strategies
+        return GridInterpolator(grid_shape, interpolation_type, **kwargs)
     raise NotImplementedError(
-        f"interpolator '{interpolation_type}' (SciPy scattered/regular-grid strategy) has no "
-        "device implementation; use 'grid' for gridded data"
+        f"interpolator '{interpolation_type}' without grid_shape (SciPy Delaunay/KD-tree strategy for "
+        "scattered points) has no device implementation; pass grid_shape for gridded data"
     )

@@ -39,10 +39,47 @@ def test_interpolator_factory():
         create_interpolator("analytical")
     with pytest.raises(ValueError, match="Invalid interpolation type"):
         create_interpolator("spline")
+    # with a grid shape the SciPy names are RegularGridInterpolator strategies (interpolate.py:895)
+    from pyftle_amd.interpolate import GridInterpolator
+
+    for name, mode in (("cubic", 3), ("Linear", 1), ("nearest", 2)):
+        g = create_interpolator(name, (8, 8))
+        assert isinstance(g, GridInterpolator) and g.mode == mode
+    assert GridInterpolator((8, 8), "slinear").mode == 1
+    with pytest.raises(ValueError, match="Invalid method"):
+        GridInterpolator((8, 8), "spline")
     with pytest.raises(NotImplementedError):
-        create_interpolator("cubic", (8, 8))
+        GridInterpolator((8, 8), "quintic")
+    with pytest.raises(ValueError, match="at least  4 points"):
+        GridInterpolator((8, 3), "cubic")
+    # without one they are the scattered-data strategies: not provided
+    with pytest.raises(NotImplementedError):
+        create_interpolator("cubic")
     with pytest.raises(ValueError, match="Unsupported grid dimensionality"):
         HighPerfInterpolator((8,))
+
+
+def test_rgi_geometry_reproduces_linspace_nodes():
+    """GridGeometry.from_rgi: node k = k*spacing + origin and the last node = max reproduce
+    numpy.linspace bit for bit (the device evaluates nodes this way), C-order strides, and
+    the reference's point-count check (interpolate.py:611-617)."""
+    from pyftle_amd.field import GridGeometry
+
+    rng = np.random.default_rng(3)
+    for n in (4, 17, 256, 1000):
+        lo, hi = rng.uniform(-5, 0), rng.uniform(0.1, 7)
+        ax = np.linspace(lo, hi, n)
+        other = np.linspace(0.0, 1.0, 5)
+        X, Y = np.meshgrid(ax, other, indexing="ij")
+        g = GridGeometry.from_rgi((n, 5), np.stack([X.ravel(), Y.ravel()]))
+        k = np.arange(n - 1, dtype=np.float64)
+        np.testing.assert_array_equal(k * g.spacing[0] + g.origin[0], ax[:-1])
+        assert g.upper[0] == ax[-1]
+        assert g.src_strides == (5 * n, 5, 1, 0)
+    g3 = GridGeometry.from_rgi((4, 5, 6), np.zeros((3, 120)) + np.arange(120.0))
+    assert g3.n == (4, 5, 6) and g3.src_strides == (120, 30, 6, 1)
+    with pytest.raises(ValueError, match="implies 20 points, but got 21"):
+        GridGeometry.from_rgi((4, 5), np.zeros((2, 21)))
 
 
 def test_analytical_interpolator_time_is_frozen_at_zero():

@@ -217,3 +217,37 @@ def test_time_linear_blend_exact_for_linear_in_time():
     np.testing.assert_allclose(vat(0.5)(q), 1.5)
     np.testing.assert_allclose(vat(2.0)(q), 3.0)
     np.testing.assert_allclose(vat(5.0)(q), 3.0)  # clamped
+
+
+def test_rgi_field_oracle_closed_forms():
+    """RGIField (the reference's GridInterpolator calls, interpolate.py:546-762):
+    multilinear fields are exact under linear, tensor cubics under cubic with a direct
+    solver, nearest returns grid values, outside -> 0, upper edge inside, NaN -> NaN."""
+    from scipy.sparse.linalg import spsolve
+
+    nx, ny, nz = 9, 7, 6
+    x, y, z = np.linspace(-1, 2, nx), np.linspace(0, 1, ny), np.linspace(0.5, 1.5, nz)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel()])
+    lin = lambda p, q: 2 + 3 * p - q + 0.5 * p * q  # noqa: E731
+    cub = lambda p, q: p**3 - 2 * q**3 + p * q**2 + p**2 * q  # noqa: E731
+    vel = np.stack([lin(X, Y).ravel(), cub(X, Y).ravel()])
+    q = np.random.default_rng(0).uniform([-1, 0], [2, 1], (200, 2))
+    v = O.RGIField((nx, ny), vel, pts, "linear").interpolate(q)
+    np.testing.assert_allclose(v[:, 0], lin(q[:, 0], q[:, 1]), atol=1e-13)
+    v = O.RGIField((nx, ny), vel, pts, "cubic", solver=spsolve).interpolate(q)
+    np.testing.assert_allclose(v[:, 1], cub(q[:, 0], q[:, 1]), atol=1e-12)
+    nn = O.RGIField((nx, ny), vel, pts, "nearest").interpolate(np.array([[x[3] + 0.3 * (x[4] - x[3]), y[2]]]))
+    assert nn[0, 0] == lin(x[3], y[2])
+    edge = O.RGIField((nx, ny), vel, pts, "linear").interpolate(np.array([[2.0, 1.0], [2.0 + 1e-9, 0.5],
+                                                                           [np.nan, 0.5]]))
+    np.testing.assert_allclose(edge[0, 0], lin(2.0, 1.0), rtol=1e-14)
+    assert edge[1, 0] == 0.0 and np.isnan(edge[2, 0])
+    # 3D: trilinear exact
+    X3, Y3, Z3 = np.meshgrid(x, y, z, indexing="ij")
+    f3 = lambda a, b, c: 1 + a - 2 * b + 3 * c + a * b * c  # noqa: E731
+    vel3 = np.stack([f3(X3, Y3, Z3).ravel()] * 3)
+    pts3 = np.stack([X3.ravel(), Y3.ravel(), Z3.ravel()])
+    q3 = np.random.default_rng(1).uniform([-1, 0, 0.5], [2, 1, 1.5], (100, 3))
+    v3 = O.RGIField((nx, ny, nz), vel3, pts3, "linear").interpolate(q3)
+    np.testing.assert_allclose(v3[:, 2], f3(q3[:, 0], q3[:, 1], q3[:, 2]), atol=1e-13)
