@@ -689,7 +689,7 @@ def cpu_baseline(wl, budget_s=20.0):
     path.rk4_steps(sub, wl["h"], wl["steps"])
     el = time.perf_counter() - t0
     seed_steps = len(idx) * wl["steps"] / el
-    return {
+    out = {
         "value": seed_steps / wl["steps"],
         "unit": "seeds/s",
         "cores": 1,
@@ -700,6 +700,55 @@ def cpu_baseline(wl, budget_s=20.0):
         "seed_steps_per_s": seed_steps,
         "cpu_model": _cpu_model(),
         "host_nproc": os.cpu_count(),
+    }
+    omp = _cpu_baseline_openmp(wl, comps, path.mins, path.deltas, pos, n, nb, budget_s / 2)
+    if omp is not None:
+        out["openmp"] = omp
+    return out
+
+
+def _cpu_baseline_openmp(wl, comps, mins, deltas, pos, n, nb, budget_s):
+    """The same workload on all host cores: the oracle's C restatement of the reference
+    CPU path (cpu_path.c: per-component index conversion and scalar multilinear loop,
+    RK4 with full stage arrays), OpenMP over the particles."""
+    import numpy as np
+
+    from oracle import native
+
+    try:
+        native.cpu_path()
+    except RuntimeError:
+        return None
+    threads = max(1, min(16, os.cpu_count() or 1))  # the GPU box's CPU share is 16 cores
+
+    class _Field:  # the attributes cpu_advect reads
+        pass
+
+    f = _Field()
+    f.dim, f.v, f.mins, f.deltas = len(comps), [np.ascontiguousarray(c) for c in comps], list(mins), list(deltas)
+    rng = np.random.default_rng(12)
+
+    def sample(k):
+        idx = np.sort(rng.choice(n, size=min(k, n), replace=False))
+        return idx, np.ascontiguousarray(np.concatenate([pos[j * n + idx] for j in range(nb)], axis=0))
+
+    idx, sub = sample(16384)
+    t0 = time.perf_counter()
+    native.cpu_advect([f], sub, 0.0, wl["h"], 25, "rk4", mode=1, threads=threads)
+    rate = len(idx) * 25 / (time.perf_counter() - t0)
+    k = int(max(1024, min(n, budget_s * rate / wl["steps"])))
+    idx, sub = sample(k)
+    t0 = time.perf_counter()
+    native.cpu_advect([f], sub, 0.0, wl["h"], wl["steps"], "rk4", mode=1, threads=threads)
+    el = time.perf_counter() - t0
+    return {
+        "value": len(idx) / el,
+        "unit": "seeds/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{len(idx)} random seeds x the full {wl['steps']}-step window ({el:.1f} s, "
+                  f"{threads} OpenMP threads): the C restatement of the reference CPU path "
+                  f"(oracle/cpu_path.c, -O3), same frozen field",
     }
 
 
