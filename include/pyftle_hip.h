@@ -22,6 +22,9 @@
  *   interpolate.py:546-762  GridInterpolator (SciPy RegularGridInterpolator, methods
  *                           linear / nearest / cubic) -> pyftle_field.interp modes,
  *                           pyftle_pack_spline (cubic coefficients)
+ *   interpolate.py:188-294  LinearInterpolator / NearestNeighborInterpolator (SciPy
+ *                           LinearNDInterpolator, NearestNDInterpolator on scattered
+ *                           points) -> PYFTLE_INTERP_SCATTER_* with pyftle_scatter
  */
 #ifndef PYFTLE_HIP_H
 #define PYFTLE_HIP_H
@@ -67,6 +70,34 @@ extern "C" {
 #define PYFTLE_INTERP_RGI_NEAREST 2 /* method="nearest": nearest node, ties to the lower node */
 #define PYFTLE_INTERP_RGI_CUBIC 3   /* method="cubic": tensor-product not-a-knot cubic spline; the
                                        slot holds B-spline coefficients (pyftle_pack_spline) */
+#define PYFTLE_INTERP_SCATTER_LINEAR 4  /* LinearNDInterpolator (interpolate.py:188-246): barycentric
+                                           blend in the Delaunay simplex holding the point, NaN outside
+                                           the convex hull; the slot holds one record per data point */
+#define PYFTLE_INTERP_SCATTER_NEAREST 5 /* NearestNDInterpolator (interpolate.py:249-294): the record
+                                           of the nearest data point (ties: lower index) */
+
+/* Scattered data (the two SCATTER modes): device arrays, described on the host.
+ * Simplices and points are indexed through uniform bucket grids (CSR lists):
+ * bucket (bx,by,bz) covers [origin + b*size, origin + (b+1)*size) per axis and is
+ * entry bx + dims[0]*(by + dims[1]*bz). */
+typedef struct pyftle_scatter {
+  int64_t n_points;            /* data points = records per slot */
+  const double *points;        /* (n_points, dim) FP64 (NEAREST) */
+  int64_t n_simplices;
+  const int32_t *simplices;    /* (n_simplices, dim+1) point indices (LINEAR) */
+  const double *transform;     /* (n_simplices, dim+1, dim): scipy Delaunay.transform (LINEAR) */
+  const int32_t *hull;         /* (n_simplices,) 1 if a facet lies on the convex hull (LINEAR) */
+  int32_t simplex_dims[3];     /* simplex bucket grid */
+  double simplex_origin[3];
+  double simplex_size[3];
+  const int32_t *simplex_offsets; /* (buckets + 1) */
+  const int32_t *simplex_ids;
+  int32_t point_dims[3];       /* point bucket grid (NEAREST) */
+  double point_origin[3];
+  double point_size[3];
+  const int32_t *point_offsets;   /* (buckets + 1) */
+  const int32_t *point_ids;
+} pyftle_scatter;
 
 /* ---- work-to-lane order for grouped kernels ---- */
 #define PYFTLE_ORDER_SEED_MAJOR 0     /* the nb particles of a seed sit in adjacent lanes */
@@ -103,6 +134,7 @@ typedef struct pyftle_field {
   int32_t interp;       /* PYFTLE_INTERP_* (0 = GridInterp.cpp semantics) */
   int32_t reserved;     /* 0 */
   double upper[3];      /* grid maximum per axis (last linspace node), RGI modes */
+  const pyftle_scatter *scatter; /* SCATTER modes (n[0] = n_points, n[1] = n[2] = 1); else NULL */
 } pyftle_field;
 
 typedef struct pyftle_advect_opts {

@@ -467,3 +467,39 @@ class RGIField:
         for c in range(self.dim):
             out[:, c] = self.comps[c](new_points)
         return out
+
+
+# ---------------------------------------------------------------------------
+# Scattered-data strategies: interpolate.py:188-294 (SciPy LinearNDInterpolator /
+# NearestNDInterpolator on the data points)
+# ---------------------------------------------------------------------------
+class ScatteredField:
+    """The reference's ``LinearInterpolator`` / ``NearestNeighborInterpolator`` after an
+    update: ``u + i v`` through one complex interpolator (interpolate.py:216-220, 272-273),
+    ``w`` through a second real one in 3D (interpolate.py:226-229, 274-277); ``interpolate``
+    fills ``(M, dim)`` (interpolate.py:231-246, 279-294).  The reference fills ``w`` only
+    when exactly three points are queried (it tests ``len(new_points) == 3``); the oracle
+    always fills it."""
+
+    def __init__(self, points, velocities, method="linear"):
+        from scipy.interpolate import LinearNDInterpolator, NearestNDInterpolator
+
+        pts = np.asarray(points, dtype=np.float64)
+        vel = np.asarray(velocities, dtype=np.float64)
+        self.dim = pts.shape[0]
+        cls = LinearNDInterpolator if method == "linear" else NearestNDInterpolator
+        self.uv = cls(pts.T, vel[0] + 1j * vel[1])
+        self.w = None
+        if self.dim == 3:
+            self.w = (LinearNDInterpolator(self.uv.tri, vel[2]) if method == "linear"
+                      else NearestNDInterpolator(pts.T, vel[2]))
+
+    def interpolate(self, new_points):
+        new_points = np.asarray(new_points, dtype=np.float64)
+        out = np.empty_like(new_points)
+        uv = self.uv(new_points)
+        out[:, 0] = uv.real
+        out[:, 1] = uv.imag
+        if self.w is not None:
+            out[:, 2] = self.w(new_points)
+        return out

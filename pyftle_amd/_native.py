@@ -20,6 +20,7 @@ TIME_STEADY, TIME_FROZEN, TIME_LINEAR = 0, 1, 2
 ORDER_SEED_MAJOR, ORDER_NEIGHBOR_MAJOR = 0, 1
 COMPUTE_NATIVE, COMPUTE_MIXED, COMPUTE_F64 = 0, 1, 2
 INTERP_GINTERP, INTERP_RGI_LINEAR, INTERP_RGI_NEAREST, INTERP_RGI_CUBIC = 0, 1, 2, 3
+INTERP_SCATTER_LINEAR, INTERP_SCATTER_NEAREST = 4, 5
 OK, EINVAL, EHIP = 0, -1, -2
 
 
@@ -43,6 +44,30 @@ class Field(ctypes.Structure):
         ("interp", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
         ("upper", ctypes.c_double * 3),
+        ("scatter", ctypes.c_void_p),
+    ]
+
+
+class Scatter(ctypes.Structure):
+    """``pyftle_scatter`` (include/pyftle_hip.h): device arrays of a scattered-data field."""
+
+    _fields_ = [
+        ("n_points", ctypes.c_int64),
+        ("points", ctypes.c_void_p),
+        ("n_simplices", ctypes.c_int64),
+        ("simplices", ctypes.c_void_p),
+        ("transform", ctypes.c_void_p),
+        ("hull", ctypes.c_void_p),
+        ("simplex_dims", ctypes.c_int32 * 3),
+        ("simplex_origin", ctypes.c_double * 3),
+        ("simplex_size", ctypes.c_double * 3),
+        ("simplex_offsets", ctypes.c_void_p),
+        ("simplex_ids", ctypes.c_void_p),
+        ("point_dims", ctypes.c_int32 * 3),
+        ("point_origin", ctypes.c_double * 3),
+        ("point_size", ctypes.c_double * 3),
+        ("point_offsets", ctypes.c_void_p),
+        ("point_ids", ctypes.c_void_p),
     ]
 
 

@@ -22,6 +22,7 @@
 
 #include "common.hpp"
 #include "rgi.hpp"
+#include "scatter.hpp"
 
 namespace pyftle {
 
@@ -138,7 +139,8 @@ __device__ __forceinline__ void sample(const FieldDev &f, const char *slot, doub
   }
 }
 
-// GridInterp (INTERP 0) or a RegularGridInterpolator mode (FP64 positions and blend).
+// GridInterp (INTERP 0), a RegularGridInterpolator mode or a scattered-data mode
+// (the last two with FP64 positions and blend).
 template <int DIM, bool PAIR, bool STORE64, int INTERP, typename Pos, typename Blend>
 __device__ __forceinline__ void sample_any(const FieldDev &f, const char *slot, double w, const Pos *x,
                                            Blend *v) {
@@ -146,8 +148,12 @@ __device__ __forceinline__ void sample_any(const FieldDev &f, const char *slot, 
     sample<DIM, PAIR, STORE64>(f, slot, w, x, v);
   } else {
     static_assert(std::is_same<Pos, double>::value && std::is_same<Blend, double>::value,
-                  "RegularGridInterpolator modes run in FP64");
-    sample_rgi<DIM, INTERP, typename std::conditional<STORE64, double, float>::type>(f, slot, x, v);
+                  "RegularGridInterpolator and scattered-data modes run in FP64");
+    using T = typename std::conditional<STORE64, double, float>::type;
+    if constexpr (INTERP == PYFTLE_INTERP_SCATTER_LINEAR || INTERP == PYFTLE_INTERP_SCATTER_NEAREST)
+      sample_scatter<DIM, INTERP, T>(f, slot, x, v);
+    else
+      sample_rgi<DIM, INTERP, T>(f, slot, x, v);
   }
 }
 

@@ -1,6 +1,7 @@
 // Instantiations of the general advection kernel for the RegularGridInterpolator
-// modes (GridInterpolator, interpolate.py:546-762): FP64 positions and blend,
-// FP32 or FP64 records, STEADY or FROZEN time, SINGLE layout.
+// modes (GridInterpolator, interpolate.py:546-762) and the scattered-data modes
+// (interpolate.py:188-294): FP64 positions and blend, FP32 or FP64 records,
+// STEADY or FROZEN time, SINGLE layout.
 #include "advect_impl.hpp"
 
 namespace pyftle {
@@ -37,7 +38,9 @@ template <int DIM>
 static void rgi_interp(int interp, const AdvectArgs &a, const LaunchSpec &sp) {
   if (interp == PYFTLE_INTERP_RGI_LINEAR) rgi_method<DIM, PYFTLE_INTERP_RGI_LINEAR>(a, sp);
   else if (interp == PYFTLE_INTERP_RGI_NEAREST) rgi_method<DIM, PYFTLE_INTERP_RGI_NEAREST>(a, sp);
-  else rgi_method<DIM, PYFTLE_INTERP_RGI_CUBIC>(a, sp);
+  else if (interp == PYFTLE_INTERP_RGI_CUBIC) rgi_method<DIM, PYFTLE_INTERP_RGI_CUBIC>(a, sp);
+  else if (interp == PYFTLE_INTERP_SCATTER_LINEAR) rgi_method<DIM, PYFTLE_INTERP_SCATTER_LINEAR>(a, sp);
+  else rgi_method<DIM, PYFTLE_INTERP_SCATTER_NEAREST>(a, sp);
 }
 
 void launch_advect_rgi(int dim, int interp, const AdvectArgs &a, const LaunchSpec &sp) {

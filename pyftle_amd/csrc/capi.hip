@@ -25,10 +25,42 @@ int check_launch(const char *what) {
   return PYFTLE_OK;
 }
 
+static int validate_scatter(const pyftle_field *f) {
+  const pyftle_scatter *sc = f->scatter;
+  if (!sc) return fail("scattered-data modes need field.scatter");
+  if (f->layout != PYFTLE_LAYOUT_SINGLE) return fail("scattered-data modes need the SINGLE layout");
+  if (sc->n_points < 1 || sc->n_points >= (int64_t)1 << 31) return fail("scatter.n_points out of range");
+  if (f->n[0] != sc->n_points || f->n[1] != 1 || (f->dim == 3 && f->n[2] != 1))
+    return fail("scattered-data fields have n = (n_points, 1, 1)");
+  const int32_t *dims = f->interp == PYFTLE_INTERP_SCATTER_LINEAR ? sc->simplex_dims : sc->point_dims;
+  const double *size = f->interp == PYFTLE_INTERP_SCATTER_LINEAR ? sc->simplex_size : sc->point_size;
+  int64_t buckets = 1;
+  for (int a = 0; a < f->dim; ++a) {
+    if (dims[a] < 1) return fail("scatter bucket grid dims must be >= 1");
+    if (!(size[a] > 0.0) || !isfinite(size[a])) return fail("scatter bucket sizes must be finite and > 0");
+    buckets *= dims[a];
+  }
+  if (buckets >= (int64_t)1 << 31) return fail("scatter bucket grid too large");
+  if (f->interp == PYFTLE_INTERP_SCATTER_LINEAR) {
+    if (sc->n_simplices < 1 || !sc->simplices || !sc->transform || !sc->hull || !sc->simplex_offsets ||
+        !sc->simplex_ids)
+      return fail("LINEAR scatter needs simplices, transform, hull flags and the simplex buckets");
+  } else {
+    if (!sc->points || !sc->point_offsets || !sc->point_ids)
+      return fail("NEAREST scatter needs the points and the point buckets");
+  }
+  return PYFTLE_OK;
+}
+
 int validate_field(const pyftle_field *f) {
   if (!f) return fail("field is NULL");
   if (f->dim != 2 && f->dim != 3) return fail("field.dim must be 2 or 3");
-  for (int a = 0; a < f->dim; ++a) {
+  const bool scatter = f->interp == PYFTLE_INTERP_SCATTER_LINEAR || f->interp == PYFTLE_INTERP_SCATTER_NEAREST;
+  if (scatter) {
+    int rc = validate_scatter(f);
+    if (rc) return rc;
+  }
+  for (int a = 0; a < f->dim && !scatter; ++a) {
     if (f->n[a] < 2) return fail("field.n must be >= 2 on every axis");
     if (!(f->spacing[a] != 0.0) || !isfinite(f->spacing[a]))
       return fail("field.spacing must be finite and non-zero");
@@ -37,9 +69,9 @@ int validate_field(const pyftle_field *f) {
   if (f->dtype != PYFTLE_F32 && f->dtype != PYFTLE_F64) return fail("field.dtype must be F32 or F64");
   if (f->layout != PYFTLE_LAYOUT_SINGLE && f->layout != PYFTLE_LAYOUT_PAIR)
     return fail("field.layout must be SINGLE or PAIR");
-  if (f->interp < PYFTLE_INTERP_GINTERP || f->interp > PYFTLE_INTERP_RGI_CUBIC)
+  if (f->interp < PYFTLE_INTERP_GINTERP || f->interp > PYFTLE_INTERP_SCATTER_NEAREST)
     return fail("field.interp must be one of PYFTLE_INTERP_*");
-  if (f->interp != PYFTLE_INTERP_GINTERP) {
+  if (f->interp != PYFTLE_INTERP_GINTERP && !scatter) {
     if (f->layout != PYFTLE_LAYOUT_SINGLE) return fail("RegularGridInterpolator modes need the SINGLE layout");
     for (int a = 0; a < f->dim; ++a) {
       if (f->periodic[a]) return fail("RegularGridInterpolator modes have no periodic axes");
@@ -54,7 +86,7 @@ int validate_field(const pyftle_field *f) {
   if (!f->data) return fail("field.data is NULL");
   int64_t nodes = padded_nodes(f);
   if (nodes > (int64_t)INT32_MAX / 2) return fail("field grid too large (> 2^30 nodes)");
-  if (axis_records(f, 0) >= (1 << 24) || (int64_t)axis_records(f, 0) * axis_records(f, 1) >= (1 << 24))
+  if (!scatter && (axis_records(f, 0) >= (1 << 24) || (int64_t)axis_records(f, 0) * axis_records(f, 1) >= (1 << 24)))
     return fail("field rows / slices must hold fewer than 2^24 nodes");
   int64_t need = nodes * record_bytes(f->dim, f->layout, f->dtype);
   if (f->slot_stride < need) return fail("field.slot_stride smaller than one slot of records");
@@ -96,6 +128,39 @@ int make_field_dev(const pyftle_field *f, FieldDev *d) {
   d->dts = f->dt_snap;
   d->interp = f->interp;
   for (int a = 0; a < 3; ++a) d->upper[a] = a < f->dim ? f->upper[a] : 0.0;
+  d->points = nullptr;
+  d->simplices = nullptr;
+  d->transform = nullptr;
+  d->hull = nullptr;
+  d->s_off = d->s_ids = d->p_off = d->p_ids = nullptr;
+  d->npts = 0;
+  for (int a = 0; a < 3; ++a) {
+    d->s_dims[a] = d->p_dims[a] = 1;
+    d->s_org[a] = d->p_org[a] = 0.0;
+    d->s_inv[a] = d->p_inv[a] = 1.0;
+    d->p_size[a] = 1.0;
+  }
+  if (f->scatter && (f->interp == PYFTLE_INTERP_SCATTER_LINEAR || f->interp == PYFTLE_INTERP_SCATTER_NEAREST)) {
+    const pyftle_scatter *sc = f->scatter;
+    d->points = sc->points;
+    d->simplices = sc->simplices;
+    d->transform = sc->transform;
+    d->hull = sc->hull;
+    d->s_off = sc->simplex_offsets;
+    d->s_ids = sc->simplex_ids;
+    d->p_off = sc->point_offsets;
+    d->p_ids = sc->point_ids;
+    d->npts = sc->n_points;
+    for (int a = 0; a < f->dim; ++a) {
+      d->s_dims[a] = sc->simplex_dims[a] > 0 ? sc->simplex_dims[a] : 1;
+      d->s_org[a] = sc->simplex_origin[a];
+      d->s_inv[a] = sc->simplex_size[a] > 0.0 ? 1.0 / sc->simplex_size[a] : 1.0;
+      d->p_dims[a] = sc->point_dims[a] > 0 ? sc->point_dims[a] : 1;
+      d->p_org[a] = sc->point_origin[a];
+      d->p_size[a] = sc->point_size[a] > 0.0 ? sc->point_size[a] : 1.0;
+      d->p_inv[a] = 1.0 / d->p_size[a];
+    }
+  }
   return PYFTLE_OK;
 }
 

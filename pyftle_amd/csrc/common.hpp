@@ -49,7 +49,21 @@ struct FieldDev {
   double t0, dts;
   int interp;       // PYFTLE_INTERP_*
   double upper[3];  // last grid node per axis (RGI modes)
+  // scattered-data modes (pyftle_scatter)
+  const double *points;
+  const int *simplices;
+  const double *transform;
+  const int *hull;
+  const int *s_off, *s_ids;
+  const int *p_off, *p_ids;
+  int64_t npts;
+  int s_dims[3], p_dims[3];
+  double s_org[3], s_inv[3], p_org[3], p_size[3], p_inv[3];
 };
+
+inline bool is_scatter(int interp) {
+  return interp == PYFTLE_INTERP_SCATTER_LINEAR || interp == PYFTLE_INTERP_SCATTER_NEAREST;
+}
 
 int make_field_dev(const pyftle_field *f, FieldDev *out);
 int validate_field(const pyftle_field *f);
@@ -63,6 +77,8 @@ inline int axis_records(const pyftle_field *f, int a) {
 
 // records of one slot
 inline int64_t padded_nodes(const pyftle_field *f) {
+  if (f->interp == PYFTLE_INTERP_SCATTER_LINEAR || f->interp == PYFTLE_INTERP_SCATTER_NEAREST)
+    return f->scatter ? f->scatter->n_points : 0;
   int64_t g = 1;
   for (int a = 0; a < f->dim; ++a) g *= (int64_t)axis_records(f, a);
   return g;

@@ -4,6 +4,7 @@
 
 #include "common.hpp"
 #include "rgi.hpp"
+#include "scatter.hpp"
 
 namespace pyftle {
 
@@ -113,7 +114,8 @@ __global__ void __launch_bounds__(256) sample_kernel(FieldDev f, const char *slo
   for (int a = 0; a < DIM; ++a) out[i * DIM + a] = (P)v[a];
 }
 
-// RegularGridInterpolator modes (rgi.hpp), one lane per point, FP64 arithmetic.
+// RegularGridInterpolator modes (rgi.hpp) and scattered-data modes (scatter.hpp),
+// one lane per point, FP64 arithmetic.
 template <int DIM, int INTERP, typename T, typename P>
 __global__ void __launch_bounds__(256) sample_rgi_kernel(FieldDev f, const char *slot,
                                                          const P *__restrict__ pts, int64_t n,
@@ -123,7 +125,10 @@ __global__ void __launch_bounds__(256) sample_rgi_kernel(FieldDev f, const char 
   double x[DIM], v[DIM];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) x[a] = (double)pts[i * DIM + a];
-  sample_rgi<DIM, INTERP, T>(f, slot, x, v);
+  if constexpr (INTERP == PYFTLE_INTERP_SCATTER_LINEAR || INTERP == PYFTLE_INTERP_SCATTER_NEAREST)
+    sample_scatter<DIM, INTERP, T>(f, slot, x, v);
+  else
+    sample_rgi<DIM, INTERP, T>(f, slot, x, v);
 #pragma unroll
   for (int a = 0; a < DIM; ++a) out[i * DIM + a] = (P)v[a];
 }
@@ -155,8 +160,12 @@ static void launch_sample_rgi(const FieldDev &d, bool f64, const char *slot, con
     launch_sample_rgi_i<DIM, PYFTLE_INTERP_RGI_LINEAR>(d, f64, slot, pts, pd, n, out, st);
   else if (d.interp == PYFTLE_INTERP_RGI_NEAREST)
     launch_sample_rgi_i<DIM, PYFTLE_INTERP_RGI_NEAREST>(d, f64, slot, pts, pd, n, out, st);
-  else
+  else if (d.interp == PYFTLE_INTERP_RGI_CUBIC)
     launch_sample_rgi_i<DIM, PYFTLE_INTERP_RGI_CUBIC>(d, f64, slot, pts, pd, n, out, st);
+  else if (d.interp == PYFTLE_INTERP_SCATTER_LINEAR)
+    launch_sample_rgi_i<DIM, PYFTLE_INTERP_SCATTER_LINEAR>(d, f64, slot, pts, pd, n, out, st);
+  else
+    launch_sample_rgi_i<DIM, PYFTLE_INTERP_SCATTER_NEAREST>(d, f64, slot, pts, pd, n, out, st);
 }
 
 template <int DIM, bool PAIR, typename Real>

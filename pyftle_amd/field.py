@@ -208,6 +208,14 @@ class GridGeometry:
                             grid_shape, tuple(upper))
 
     @staticmethod
+    def scattered(dim: int, n_points: int) -> "GridGeometry":
+        """Record layout of a scattered-data field: one record per data point, the
+        snapshot's ``(dim, N)`` components read point by point."""
+        n = (int(n_points),) + (1,) * (dim - 1)
+        return GridGeometry(dim, n, (0.0,) * dim, (1.0,) * dim, (False,) * dim, (int(n_points), 1, 0, 0),
+                            (int(n_points),))
+
+    @staticmethod
     def uniform(n, origin, spacing, periodic=None) -> "GridGeometry":
         """Geometry of a grid given directly in device order (x fastest).
 
@@ -268,9 +276,10 @@ class DeviceField:
 
     def __init__(self, geom: GridGeometry, *, dtype=torch.float32, layout=N.LAYOUT_SINGLE,
                  ring_slots: int = 1, n_snapshots: int = 1, t0: float = 0.0, dt_snap: float = 1.0,
-                 interp: int = N.INTERP_GINTERP):
+                 interp: int = N.INTERP_GINTERP, scatter=None):
         self.geom = geom
         self.interp = int(interp)
+        self.scatter = scatter  # ScatterData of the scattered-data modes (kept alive with the field)
         self.dtype = precision_dtype(dtype)
         self.layout = layout
         self.ring_slots = int(ring_slots)
@@ -320,6 +329,7 @@ class DeviceField:
         d.interp = self.interp
         for a in range(3):
             d.upper[a] = g.upper[a] if a < g.dim and len(g.upper) > a else 0.0
+        d.scatter = self.scatter.address if self.scatter is not None else None
         return d
 
     @property

@@ -16,13 +16,16 @@ Differences from the reference, by design:
 * ``cubic``, ``linear`` and ``nearest`` with a ``grid_shape`` are the SciPy
   ``RegularGridInterpolator`` strategies (``GridInterpolator``,
   interpolate.py:546-762), run by the same device kernels in their own
-  interpolation modes.  Without a grid shape they are SciPy's scattered-data
-  strategies (Delaunay / KD-tree, interpolate.py:114-294), which have no
-  device implementation; the factory raises ``NotImplementedError``.
+  interpolation modes.  Without a grid shape, ``linear`` and ``nearest`` are
+  SciPy's scattered-data strategies (``LinearInterpolator``: Delaunay +
+  barycentric; ``NearestNeighborInterpolator``), also on the device
+  (interpolate.py:188-294); ``cubic`` there (Clough-Tocher) is not provided and
+  the factory raises ``NotImplementedError``.
 """
 
 from __future__ import annotations
 
+import hashlib
 from abc import ABC, abstractmethod
 from typing import Callable, Optional
 
@@ -225,6 +228,94 @@ class GridInterpolator(HighPerfInterpolator):
             self._allocate()
 
 
+class _ScatteredInterpolator(HighPerfInterpolator):
+    """Scattered-data strategies on the device (interpolate.py:188-294): the data
+    points are triangulated / bucketed once per point set (``ScatterData``, cached
+    by content), each snapshot's velocities become one record per point in the
+    device ring, and the kernels search and blend on the GPU."""
+
+    _mode = N.INTERP_SCATTER_LINEAR
+
+    def __init__(self, precision="float32", ring_slots: int = 1, max_ring_bytes: int = 8 << 30):
+        Interpolator.__init__(self)
+        self.dim = None
+        self.dtype = precision_dtype(precision)
+        self.periodic = None
+        self.geom: Optional[GridGeometry] = None
+        self.field: Optional[DeviceField] = None
+        self._ring = max(1, int(ring_slots))
+        self.max_ring_bytes = int(max_ring_bytes)
+        self.current = -1
+        self._scatter = None
+        self._points_key = None
+
+    def slot_bytes(self) -> int:
+        if self.geom is None:
+            return 1
+        comps = 2 if self.dim == 2 else 4
+        esize = 4 if self.dtype == torch.float32 else 8
+        return self.geom.n[0] * comps * esize
+
+    def _allocate(self) -> None:
+        self.field = DeviceField(self.geom, dtype=self.dtype, layout=N.LAYOUT_SINGLE, ring_slots=self._ring,
+                                 n_snapshots=1 << 30, interp=self._mode, scatter=self._scatter)
+
+    def _initialize_interpolator(self) -> None:
+        if self.velocities is None or self.points is None:
+            raise ValueError("Velocities and points must be set before initialization.")
+        pts = self.points.detach().cpu().numpy() if isinstance(self.points, torch.Tensor) else np.asarray(self.points)
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        if pts.ndim != 2 or pts.shape[0] not in (2, 3):
+            raise ValueError("points must have shape (dim, N) with dim 2 or 3")
+        if np.shape(self.velocities)[-1] != pts.shape[-1]:
+            raise ValueError("Number of velocities must match the number of points.")
+        self._check_dim(pts.shape[0])
+        key = (pts.shape, hashlib.sha1(pts.tobytes()).hexdigest())
+        if key != self._points_key:
+            from .scatter import ScatterData
+
+            self._scatter = ScatterData(pts, need_simplices=self._mode == N.INTERP_SCATTER_LINEAR)
+            self._points_key = key
+            self.dim = pts.shape[0]
+            self.geom = GridGeometry.scattered(self.dim, pts.shape[1])
+            self._allocate()
+
+    def _check_dim(self, dim: int) -> None:
+        pass
+
+    def update(self, velocities, points=None) -> None:
+        """interpolate.py:76-93: a new snapshot (and point set) for the next ring slot."""
+        self.velocities = velocities
+        if points is not None:
+            self.points = points
+        if self.points is None:
+            raise ValueError("Velocities and points must be set before initialization.")
+        self._initialize_interpolator()
+        n = self.geom.n[0]
+        if tuple(np.shape(velocities)) != (self.dim, n):
+            raise ValueError(f"velocities must have shape {(self.dim, n)}, got {tuple(np.shape(velocities))}")
+        self.current += 1
+        self.field.pack(self.current, velocities)
+
+
+class LinearInterpolator(_ScatteredInterpolator):
+    """``LinearNDInterpolator`` on the Delaunay triangulation of the data points
+    (interpolate.py:188-246): barycentric blend in the containing simplex, NaN
+    outside the convex hull.  The reference fills the third component in 3D only
+    when exactly three points are queried (interpolate.py:244 tests the number of
+    points, not the dimension) and leaves it uninitialised otherwise; here all
+    components are always interpolated."""
+
+    _mode = N.INTERP_SCATTER_LINEAR
+
+
+class NearestNeighborInterpolator(_ScatteredInterpolator):
+    """``NearestNDInterpolator`` (interpolate.py:249-294): the velocity of the
+    nearest data point (ties: the lower point index), defined everywhere."""
+
+    _mode = N.INTERP_SCATTER_NEAREST
+
+
 class AnalyticalInterpolator(Interpolator):
     """interpolate.py:765-829: ``velocity_fn(time, points)`` with time fixed at 0.
 
@@ -298,7 +389,11 @@ def create_interpolator(interpolation_type: str, grid_shape=None, velocity_fn=No
         return HighPerfInterpolator(grid_shape, **kwargs)
     if grid_shape is not None:  # interpolate.py:895-896: RegularGridInterpolator strategies
         return GridInterpolator(grid_shape, interpolation_type, **kwargs)
+    if interpolation_type == "linear":
+        return LinearInterpolator(**kwargs)
+    if interpolation_type == "nearest":
+        return NearestNeighborInterpolator(**kwargs)
     raise NotImplementedError(
-        f"interpolator '{interpolation_type}' without grid_shape (SciPy Delaunay/KD-tree strategy for "
-        "scattered points) has no device implementation; pass grid_shape for gridded data"
+        "interpolator 'cubic' without grid_shape (SciPy Clough-Tocher on scattered points) has no "
+        "device implementation; pass grid_shape for gridded data"
     )

@@ -385,3 +385,30 @@ def test_vtk_writer_arrays_follow_the_reference_layout(tmp_path):
     np.testing.assert_array_equal(a["offsets"], np.arange(1, 5))
     with pytest.raises(ValueError, match="Invalid grid_shape"):
         VTKWriter(str(tmp_path / "bad"), (2, 2, 2, 2)).write("x", np.zeros(16), np.zeros((16, 3)))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_scatter_host_structures(dim):
+    """Bucket grids of the scattered-data modes: every point sits in exactly one bucket,
+    and every simplex that contains a query point is listed in that point's bucket (so
+    the device's first-fit search over the bucket cannot miss it)."""
+    from scipy.spatial import Delaunay
+
+    from pyftle_amd.scatter import build_host
+
+    rng = np.random.default_rng(dim)
+    pts = rng.uniform(-1.0, 2.0, (dim, 400 if dim == 2 else 300))
+    h = build_host(pts)
+    ids = np.sort(h["p_ids"])
+    np.testing.assert_array_equal(ids, np.arange(pts.shape[1]))
+    assert h["p_off"][-1] == pts.shape[1] and h["s_off"][-1] == h["s_ids"].size
+    tri = Delaunay(pts.T)
+    np.testing.assert_array_equal(h["simplices"], tri.simplices)
+    q = rng.uniform(-1.0, 2.0, (2000, dim))
+    s = tri.find_simplex(q)
+    for x, si in zip(q[s >= 0], s[s >= 0]):
+        k = np.clip(np.floor((x - h["s_org"]) / h["s_size"]).astype(int), 0, h["s_dims"] - 1)
+        b = k[0] + h["s_dims"][0] * (k[1] + (h["s_dims"][1] * k[2] if dim == 3 else 0))
+        assert si in h["s_ids"][h["s_off"][b]:h["s_off"][b + 1]]
+    # hull flags: simplices with a neighbour missing
+    np.testing.assert_array_equal(h["hull"], (tri.neighbors < 0).any(axis=1))
