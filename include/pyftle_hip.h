@@ -22,7 +22,8 @@
  *   interpolate.py:546-762  GridInterpolator (SciPy RegularGridInterpolator, methods
  *                           linear / nearest / cubic) -> pyftle_field.interp modes,
  *                           pyftle_pack_spline (cubic coefficients)
- *   interpolate.py:188-294  LinearInterpolator / NearestNeighborInterpolator (SciPy
+ *   interpolate.py:114-294  CubicInterpolator / LinearInterpolator /
+ *                           NearestNeighborInterpolator (SciPy CloughTocher2DInterpolator,
  *                           LinearNDInterpolator, NearestNDInterpolator on scattered
  *                           points) -> PYFTLE_INTERP_SCATTER_* with pyftle_scatter
  */
@@ -75,8 +76,14 @@ extern "C" {
                                            the convex hull; the slot holds one record per data point */
 #define PYFTLE_INTERP_SCATTER_NEAREST 5 /* NearestNDInterpolator (interpolate.py:249-294): the record
                                            of the nearest data point (ties: lower index) */
+#define PYFTLE_INTERP_SCATTER_CUBIC 6   /* CloughTocher2DInterpolator (interpolate.py:114-185), 2D: the
+                                           Clough-Tocher cubic patch of the containing triangle, NaN
+                                           outside the hull; one record per data point of 8 components
+                                           (u, v, du/dx, du/dy, dv/dx, dv/dy, 0, 0) */
 
 /* Scattered data (the two SCATTER modes): device arrays, described on the host.
+ * The arrays must be consistent (ids below their counts, offsets of length
+ * buckets + 1); the library reads them as given.
  * Simplices and points are indexed through uniform bucket grids (CSR lists):
  * bucket (bx,by,bz) covers [origin + b*size, origin + (b+1)*size) per axis and is
  * entry bx + dims[0]*(by + dims[1]*bz). */
@@ -97,6 +104,7 @@ typedef struct pyftle_scatter {
   double point_size[3];
   const int32_t *point_offsets;   /* (buckets + 1) */
   const int32_t *point_ids;
+  const double *ct_g;          /* (n_simplices, 3) cross-edge direction factors (CUBIC) */
 } pyftle_scatter;
 
 /* ---- work-to-lane order for grouped kernels ---- */

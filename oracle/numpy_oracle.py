@@ -474,7 +474,8 @@ class RGIField:
 # NearestNDInterpolator on the data points)
 # ---------------------------------------------------------------------------
 class ScatteredField:
-    """The reference's ``LinearInterpolator`` / ``NearestNeighborInterpolator`` after an
+    """The reference's ``CubicInterpolator`` (2D) / ``LinearInterpolator`` /
+    ``NearestNeighborInterpolator`` after an
     update: ``u + i v`` through one complex interpolator (interpolate.py:216-220, 272-273),
     ``w`` through a second real one in 3D (interpolate.py:226-229, 274-277); ``interpolate``
     fills ``(M, dim)`` (interpolate.py:231-246, 279-294).  The reference fills ``w`` only
@@ -482,12 +483,13 @@ class ScatteredField:
     always fills it."""
 
     def __init__(self, points, velocities, method="linear"):
-        from scipy.interpolate import LinearNDInterpolator, NearestNDInterpolator
+        from scipy.interpolate import CloughTocher2DInterpolator, LinearNDInterpolator, NearestNDInterpolator
 
         pts = np.asarray(points, dtype=np.float64)
         vel = np.asarray(velocities, dtype=np.float64)
         self.dim = pts.shape[0]
-        cls = LinearNDInterpolator if method == "linear" else NearestNDInterpolator
+        cls = {"linear": LinearNDInterpolator, "nearest": NearestNDInterpolator,
+               "cubic": CloughTocher2DInterpolator}[method]  # cubic: interpolate.py:136-155, 2D
         self.uv = cls(pts.T, vel[0] + 1j * vel[1])
         self.w = None
         if self.dim == 3:

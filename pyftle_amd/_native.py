@@ -20,7 +20,7 @@ TIME_STEADY, TIME_FROZEN, TIME_LINEAR = 0, 1, 2
 ORDER_SEED_MAJOR, ORDER_NEIGHBOR_MAJOR = 0, 1
 COMPUTE_NATIVE, COMPUTE_MIXED, COMPUTE_F64 = 0, 1, 2
 INTERP_GINTERP, INTERP_RGI_LINEAR, INTERP_RGI_NEAREST, INTERP_RGI_CUBIC = 0, 1, 2, 3
-INTERP_SCATTER_LINEAR, INTERP_SCATTER_NEAREST = 4, 5
+INTERP_SCATTER_LINEAR, INTERP_SCATTER_NEAREST, INTERP_SCATTER_CUBIC = 4, 5, 6
 OK, EINVAL, EHIP = 0, -1, -2
 
 
@@ -68,6 +68,7 @@ class Scatter(ctypes.Structure):
         ("point_size", ctypes.c_double * 3),
         ("point_offsets", ctypes.c_void_p),
         ("point_ids", ctypes.c_void_p),
+        ("ct_g", ctypes.c_void_p),
     ]
 
 

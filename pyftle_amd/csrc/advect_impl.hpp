@@ -150,7 +150,8 @@ __device__ __forceinline__ void sample_any(const FieldDev &f, const char *slot, 
     static_assert(std::is_same<Pos, double>::value && std::is_same<Blend, double>::value,
                   "RegularGridInterpolator and scattered-data modes run in FP64");
     using T = typename std::conditional<STORE64, double, float>::type;
-    if constexpr (INTERP == PYFTLE_INTERP_SCATTER_LINEAR || INTERP == PYFTLE_INTERP_SCATTER_NEAREST)
+    if constexpr (INTERP == PYFTLE_INTERP_SCATTER_LINEAR || INTERP == PYFTLE_INTERP_SCATTER_NEAREST ||
+                  INTERP == PYFTLE_INTERP_SCATTER_CUBIC)
       sample_scatter<DIM, INTERP, T>(f, slot, x, v);
     else
       sample_rgi<DIM, INTERP, T>(f, slot, x, v);

@@ -40,7 +40,8 @@ static void rgi_interp(int interp, const AdvectArgs &a, const LaunchSpec &sp) {
   else if (interp == PYFTLE_INTERP_RGI_NEAREST) rgi_method<DIM, PYFTLE_INTERP_RGI_NEAREST>(a, sp);
   else if (interp == PYFTLE_INTERP_RGI_CUBIC) rgi_method<DIM, PYFTLE_INTERP_RGI_CUBIC>(a, sp);
   else if (interp == PYFTLE_INTERP_SCATTER_LINEAR) rgi_method<DIM, PYFTLE_INTERP_SCATTER_LINEAR>(a, sp);
-  else rgi_method<DIM, PYFTLE_INTERP_SCATTER_NEAREST>(a, sp);
+  else if (interp == PYFTLE_INTERP_SCATTER_NEAREST) rgi_method<DIM, PYFTLE_INTERP_SCATTER_NEAREST>(a, sp);
+  else if constexpr (DIM == 2) rgi_method<DIM, PYFTLE_INTERP_SCATTER_CUBIC>(a, sp);
 }
 
 void launch_advect_rgi(int dim, int interp, const AdvectArgs &a, const LaunchSpec &sp) {

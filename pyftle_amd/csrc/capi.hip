@@ -32,8 +32,9 @@ static int validate_scatter(const pyftle_field *f) {
   if (sc->n_points < 1 || sc->n_points >= (int64_t)1 << 31) return fail("scatter.n_points out of range");
   if (f->n[0] != sc->n_points || f->n[1] != 1 || (f->dim == 3 && f->n[2] != 1))
     return fail("scattered-data fields have n = (n_points, 1, 1)");
-  const int32_t *dims = f->interp == PYFTLE_INTERP_SCATTER_LINEAR ? sc->simplex_dims : sc->point_dims;
-  const double *size = f->interp == PYFTLE_INTERP_SCATTER_LINEAR ? sc->simplex_size : sc->point_size;
+  const bool simplex_grid = f->interp != PYFTLE_INTERP_SCATTER_NEAREST;
+  const int32_t *dims = simplex_grid ? sc->simplex_dims : sc->point_dims;
+  const double *size = simplex_grid ? sc->simplex_size : sc->point_size;
   int64_t buckets = 1;
   for (int a = 0; a < f->dim; ++a) {
     if (dims[a] < 1) return fail("scatter bucket grid dims must be >= 1");
@@ -41,10 +42,14 @@ static int validate_scatter(const pyftle_field *f) {
     buckets *= dims[a];
   }
   if (buckets >= (int64_t)1 << 31) return fail("scatter bucket grid too large");
-  if (f->interp == PYFTLE_INTERP_SCATTER_LINEAR) {
+  if (f->interp == PYFTLE_INTERP_SCATTER_LINEAR || f->interp == PYFTLE_INTERP_SCATTER_CUBIC) {
     if (sc->n_simplices < 1 || !sc->simplices || !sc->transform || !sc->hull || !sc->simplex_offsets ||
         !sc->simplex_ids)
-      return fail("LINEAR scatter needs simplices, transform, hull flags and the simplex buckets");
+      return fail("LINEAR/CUBIC scatter needs simplices, transform, hull flags and the simplex buckets");
+    if (f->interp == PYFTLE_INTERP_SCATTER_CUBIC) {
+      if (f->dim != 2) return fail("cubic interpolator is only valid for 2D cases");
+      if (!sc->points || !sc->ct_g) return fail("CUBIC scatter needs the points and the cross-edge factors");
+    }
   } else {
     if (!sc->points || !sc->point_offsets || !sc->point_ids)
       return fail("NEAREST scatter needs the points and the point buckets");
@@ -55,7 +60,7 @@ static int validate_scatter(const pyftle_field *f) {
 int validate_field(const pyftle_field *f) {
   if (!f) return fail("field is NULL");
   if (f->dim != 2 && f->dim != 3) return fail("field.dim must be 2 or 3");
-  const bool scatter = f->interp == PYFTLE_INTERP_SCATTER_LINEAR || f->interp == PYFTLE_INTERP_SCATTER_NEAREST;
+  const bool scatter = is_scatter(f->interp);
   if (scatter) {
     int rc = validate_scatter(f);
     if (rc) return rc;
@@ -69,7 +74,7 @@ int validate_field(const pyftle_field *f) {
   if (f->dtype != PYFTLE_F32 && f->dtype != PYFTLE_F64) return fail("field.dtype must be F32 or F64");
   if (f->layout != PYFTLE_LAYOUT_SINGLE && f->layout != PYFTLE_LAYOUT_PAIR)
     return fail("field.layout must be SINGLE or PAIR");
-  if (f->interp < PYFTLE_INTERP_GINTERP || f->interp > PYFTLE_INTERP_SCATTER_NEAREST)
+  if (f->interp < PYFTLE_INTERP_GINTERP || f->interp > PYFTLE_INTERP_SCATTER_CUBIC)
     return fail("field.interp must be one of PYFTLE_INTERP_*");
   if (f->interp != PYFTLE_INTERP_GINTERP && !scatter) {
     if (f->layout != PYFTLE_LAYOUT_SINGLE) return fail("RegularGridInterpolator modes need the SINGLE layout");
@@ -88,7 +93,7 @@ int validate_field(const pyftle_field *f) {
   if (nodes > (int64_t)INT32_MAX / 2) return fail("field grid too large (> 2^30 nodes)");
   if (!scatter && (axis_records(f, 0) >= (1 << 24) || (int64_t)axis_records(f, 0) * axis_records(f, 1) >= (1 << 24)))
     return fail("field rows / slices must hold fewer than 2^24 nodes");
-  int64_t need = nodes * record_bytes(f->dim, f->layout, f->dtype);
+  int64_t need = nodes * record_bytes(f->dim, f->layout, f->dtype, f->interp);
   if (f->slot_stride < need) return fail("field.slot_stride smaller than one slot of records");
   if (f->slot_stride % 16 != 0) return fail("field.slot_stride must be a multiple of 16 bytes");
   if ((reinterpret_cast<uintptr_t>(f->data) % 32) != 0) return fail("field.data must be 32-byte aligned");
@@ -140,7 +145,8 @@ int make_field_dev(const pyftle_field *f, FieldDev *d) {
     d->s_inv[a] = d->p_inv[a] = 1.0;
     d->p_size[a] = 1.0;
   }
-  if (f->scatter && (f->interp == PYFTLE_INTERP_SCATTER_LINEAR || f->interp == PYFTLE_INTERP_SCATTER_NEAREST)) {
+  d->ct_g = nullptr;
+  if (f->scatter && is_scatter(f->interp)) {
     const pyftle_scatter *sc = f->scatter;
     d->points = sc->points;
     d->simplices = sc->simplices;
@@ -150,6 +156,7 @@ int make_field_dev(const pyftle_field *f, FieldDev *d) {
     d->s_ids = sc->simplex_ids;
     d->p_off = sc->point_offsets;
     d->p_ids = sc->point_ids;
+    d->ct_g = sc->ct_g;
     d->npts = sc->n_points;
     for (int a = 0; a < f->dim; ++a) {
       d->s_dims[a] = sc->simplex_dims[a] > 0 ? sc->simplex_dims[a] : 1;

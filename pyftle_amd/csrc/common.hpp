@@ -56,13 +56,15 @@ struct FieldDev {
   const int *hull;
   const int *s_off, *s_ids;
   const int *p_off, *p_ids;
+  const double *ct_g;
   int64_t npts;
   int s_dims[3], p_dims[3];
   double s_org[3], s_inv[3], p_org[3], p_size[3], p_inv[3];
 };
 
 inline bool is_scatter(int interp) {
-  return interp == PYFTLE_INTERP_SCATTER_LINEAR || interp == PYFTLE_INTERP_SCATTER_NEAREST;
+  return interp == PYFTLE_INTERP_SCATTER_LINEAR || interp == PYFTLE_INTERP_SCATTER_NEAREST ||
+         interp == PYFTLE_INTERP_SCATTER_CUBIC;
 }
 
 int make_field_dev(const pyftle_field *f, FieldDev *out);
@@ -77,15 +79,16 @@ inline int axis_records(const pyftle_field *f, int a) {
 
 // records of one slot
 inline int64_t padded_nodes(const pyftle_field *f) {
-  if (f->interp == PYFTLE_INTERP_SCATTER_LINEAR || f->interp == PYFTLE_INTERP_SCATTER_NEAREST)
-    return f->scatter ? f->scatter->n_points : 0;
+  if (is_scatter(f->interp)) return f->scatter ? f->scatter->n_points : 0;
   int64_t g = 1;
   for (int a = 0; a < f->dim; ++a) g *= (int64_t)axis_records(f, a);
   return g;
 }
 
-// record size in bytes for (dim, layout, dtype)
-inline int record_bytes(int dim, int layout, int dtype) {
+// record size in bytes for (dim, layout, dtype); the scattered cubic mode stores 8
+// components per point (value and gradient of u and v)
+inline int record_bytes(int dim, int layout, int dtype, int interp = PYFTLE_INTERP_GINTERP) {
+  if (interp == PYFTLE_INTERP_SCATTER_CUBIC) return 8 * (dtype == PYFTLE_F64 ? 8 : 4);
   int comps = (dim == 2 ? 2 : 4) * (layout == PYFTLE_LAYOUT_PAIR ? 2 : 1);
   return comps * (dtype == PYFTLE_F64 ? 8 : 4);
 }

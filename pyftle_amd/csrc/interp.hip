@@ -125,7 +125,8 @@ __global__ void __launch_bounds__(256) sample_rgi_kernel(FieldDev f, const char 
   double x[DIM], v[DIM];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) x[a] = (double)pts[i * DIM + a];
-  if constexpr (INTERP == PYFTLE_INTERP_SCATTER_LINEAR || INTERP == PYFTLE_INTERP_SCATTER_NEAREST)
+  if constexpr (INTERP == PYFTLE_INTERP_SCATTER_LINEAR || INTERP == PYFTLE_INTERP_SCATTER_NEAREST ||
+                INTERP == PYFTLE_INTERP_SCATTER_CUBIC)
     sample_scatter<DIM, INTERP, T>(f, slot, x, v);
   else
     sample_rgi<DIM, INTERP, T>(f, slot, x, v);
@@ -164,8 +165,10 @@ static void launch_sample_rgi(const FieldDev &d, bool f64, const char *slot, con
     launch_sample_rgi_i<DIM, PYFTLE_INTERP_RGI_CUBIC>(d, f64, slot, pts, pd, n, out, st);
   else if (d.interp == PYFTLE_INTERP_SCATTER_LINEAR)
     launch_sample_rgi_i<DIM, PYFTLE_INTERP_SCATTER_LINEAR>(d, f64, slot, pts, pd, n, out, st);
-  else
+  else if (d.interp == PYFTLE_INTERP_SCATTER_NEAREST)
     launch_sample_rgi_i<DIM, PYFTLE_INTERP_SCATTER_NEAREST>(d, f64, slot, pts, pd, n, out, st);
+  else if constexpr (DIM == 2)
+    launch_sample_rgi_i<DIM, PYFTLE_INTERP_SCATTER_CUBIC>(d, f64, slot, pts, pd, n, out, st);
 }
 
 template <int DIM, bool PAIR, typename Real>
@@ -213,6 +216,8 @@ int pyftle_pack_snapshot(const pyftle_field *field, int32_t snapshot, const void
     return fail("pack_snapshot: snapshot index out of range");
   if (field->interp == PYFTLE_INTERP_RGI_CUBIC)
     return fail("pack_snapshot: cubic-mode slots hold spline coefficients (use pyftle_pack_spline)");
+  if (field->interp == PYFTLE_INTERP_SCATTER_CUBIC)
+    return fail("pack_snapshot: scattered cubic slots hold values and gradients (write the records directly)");
   const int dim = field->dim;
   const int nz = dim == 3 ? field->n[2] : 1;
   const int per_level = dim == 2 ? 2 : 4;

@@ -1,5 +1,6 @@
-// Scattered-data strategies on the device (interpolate.py:188-294): SciPy's
-// LinearNDInterpolator on a Delaunay triangulation and NearestNDInterpolator.
+// Scattered-data strategies on the device (interpolate.py:114-294): SciPy's
+// LinearNDInterpolator on a Delaunay triangulation, NearestNDInterpolator and, in
+// 2D, CloughTocher2DInterpolator.
 //
 // LINEAR: the simplex holding x is found among those listed in x's bucket
 // (every simplex is listed in each bucket its bounding box touches); a simplex
@@ -15,6 +16,8 @@
 // best squared distance is no larger than the squared distance from x to any
 // bucket not yet searched; ties go to the lower point index.
 #pragma once
+
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -70,10 +73,9 @@ __device__ __forceinline__ bool barycentric_inside(const double *__restrict__ T,
   return in && (c[DIM] >= -eps) && (c[DIM] <= 1.0 + eps);
 }
 
-template <int DIM, typename T>
-__device__ __forceinline__ void sample_scatter_linear(const FieldDev &f, const char *slot, const double *x,
-                                                      double *out) {
-  const double nan = __longlong_as_double(0x7ff8000000000000LL);
+// Index of the simplex holding x (and its barycentric coordinates), or -1.
+template <int DIM>
+__device__ __forceinline__ int scatter_locate(const FieldDev &f, const double *x, double *c) {
   int64_t b = 0, pitch = 1;
   bool in = true;
 #pragma unroll
@@ -86,7 +88,6 @@ __device__ __forceinline__ void sample_scatter_linear(const FieldDev &f, const c
     pitch *= f.s_dims[a];
   }
   int found = -1;
-  double c[DIM + 1];
   if (in) {
     const int e0 = f.s_off[b], e1 = f.s_off[b + 1];
     for (int e = e0; e < e1 && found < 0; ++e) {
@@ -100,9 +101,17 @@ __device__ __forceinline__ void sample_scatter_linear(const FieldDev &f, const c
         found = s;
     }
   }
+  return found;
+}
+
+template <int DIM, typename T>
+__device__ __forceinline__ void sample_scatter_linear(const FieldDev &f, const char *slot, const double *x,
+                                                      double *out) {
+  double c[DIM + 1];
+  const int found = scatter_locate<DIM>(f, x, c);
   if (found < 0) {
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) out[d] = nan;
+    for (int d = 0; d < DIM; ++d) out[d] = __longlong_as_double(0x7ff8000000000000LL);
     return;
   }
 #pragma unroll
@@ -115,6 +124,91 @@ __device__ __forceinline__ void sample_scatter_linear(const FieldDev &f, const c
 #pragma unroll
     for (int d = 0; d < DIM; ++d) out[d] = out[d] + c[k] * v[d];
   }
+}
+
+// (u, v, du/dx, du/dy, dv/dx, dv/dy) of one point of a scattered cubic slot
+template <typename T>
+__device__ __forceinline__ void ct_record(const char *slot, int64_t rec, double *r) {
+  if constexpr (std::is_same<T, float>::value) {
+    const float4 *p = reinterpret_cast<const float4 *>(slot) + rec * 2;
+    const float4 a = p[0], b = p[1];
+    r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w; r[4] = b.x; r[5] = b.y;
+  } else {
+    const double2 *p = reinterpret_cast<const double2 *>(slot) + rec * 4;
+    const double2 a = p[0], b = p[1], d = p[2];
+    r[0] = a.x; r[1] = a.y; r[2] = b.x; r[3] = b.y; r[4] = d.x; r[5] = d.y;
+  }
+}
+
+// Clough-Tocher patch (Farin 1986, "Triangular Bernstein-Bezier patches"): the
+// triangle is split at its centroid into three cubic Bezier pieces; vertex values
+// and gradients fix the ordinates at and next to the vertices, the cross-edge
+// derivative along direction g (toward the neighbouring triangle's centroid,
+// g = -1/2 on the hull) is made linear along each edge, and C1 continuity inside
+// fixes the ordinates next to and at the centroid.  b: barycentric coordinates.
+__device__ __forceinline__ double ct_patch(const double *P, const double *fv, const double (*gr)[2],
+                                           const double *g, const double *b) {
+  const double e12x = P[2] - P[0], e12y = P[3] - P[1];
+  const double e13x = P[4] - P[0], e13y = P[5] - P[1];
+  const double e23x = P[4] - P[2], e23y = P[5] - P[3];
+  const double c3000 = fv[0], c0300 = fv[1], c0030 = fv[2];
+  const double c2100 = (gr[0][0] * e12x + gr[0][1] * e12y + 3.0 * c3000) / 3.0;
+  const double c2010 = (gr[0][0] * e13x + gr[0][1] * e13y + 3.0 * c3000) / 3.0;
+  const double c1200 = (-(gr[1][0] * e12x + gr[1][1] * e12y) + 3.0 * c0300) / 3.0;
+  const double c0210 = (gr[1][0] * e23x + gr[1][1] * e23y + 3.0 * c0300) / 3.0;
+  const double c1020 = (-(gr[2][0] * e13x + gr[2][1] * e13y) + 3.0 * c0030) / 3.0;
+  const double c0120 = (-(gr[2][0] * e23x + gr[2][1] * e23y) + 3.0 * c0030) / 3.0;
+  const double c2001 = (c2100 + c2010 + c3000) / 3.0;
+  const double c0201 = (c1200 + c0300 + c0210) / 3.0;
+  const double c0021 = (c1020 + c0120 + c0030) / 3.0;
+  const double c0111 = (g[0] * (-c0300 + 3.0 * c0210 - 3.0 * c0120 + c0030) +
+                        (-c0300 + 2.0 * c0210 - c0120 + c0021 + c0201)) / 2.0;
+  const double c1011 = (g[1] * (-c0030 + 3.0 * c1020 - 3.0 * c2010 + c3000) +
+                        (-c0030 + 2.0 * c1020 - c2010 + c2001 + c0021)) / 2.0;
+  const double c1101 = (g[2] * (-c3000 + 3.0 * c2100 - 3.0 * c1200 + c0300) +
+                        (-c3000 + 2.0 * c2100 - c1200 + c2001 + c0201)) / 2.0;
+  const double c1002 = (c1101 + c1011 + c2001) / 3.0;
+  const double c0102 = (c1101 + c0111 + c0201) / 3.0;
+  const double c0012 = (c1011 + c0111 + c0021) / 3.0;
+  const double c0003 = (c1002 + c0102 + c0012) / 3.0;
+  const double mn = fmin(b[0], fmin(b[1], b[2]));
+  const double b1 = b[0] - mn, b2 = b[1] - mn, b3 = b[2] - mn, b4 = 3.0 * mn;
+  return b1 * b1 * b1 * c3000 + 3.0 * b1 * b1 * b2 * c2100 + 3.0 * b1 * b1 * b3 * c2010 +
+         3.0 * b1 * b1 * b4 * c2001 + 3.0 * b1 * b2 * b2 * c1200 + 6.0 * b1 * b2 * b4 * c1101 +
+         3.0 * b1 * b3 * b3 * c1020 + 6.0 * b1 * b3 * b4 * c1011 + 3.0 * b1 * b4 * b4 * c1002 +
+         b2 * b2 * b2 * c0300 + 3.0 * b2 * b2 * b3 * c0210 + 3.0 * b2 * b2 * b4 * c0201 +
+         3.0 * b2 * b3 * b3 * c0120 + 6.0 * b2 * b3 * b4 * c0111 + 3.0 * b2 * b4 * b4 * c0102 +
+         b3 * b3 * b3 * c0030 + 3.0 * b3 * b3 * b4 * c0021 + 3.0 * b3 * b4 * b4 * c0012 +
+         b4 * b4 * b4 * c0003;
+}
+
+template <typename T>
+__device__ __forceinline__ void sample_scatter_cubic(const FieldDev &f, const char *slot, const double *x,
+                                                     double *out) {
+  double c[3];
+  const int found = scatter_locate<2>(f, x, c);
+  if (found < 0) {
+    out[0] = out[1] = __longlong_as_double(0x7ff8000000000000LL);
+    return;
+  }
+  double P[6], fu[3], fw[3], gu[3][2], gw[3][2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int vtx = f.simplices[(int64_t)found * 3 + k];
+    P[2 * k] = f.points[(int64_t)vtx * 2];
+    P[2 * k + 1] = f.points[(int64_t)vtx * 2 + 1];
+    double r[6];
+    ct_record<T>(slot, vtx, r);
+    fu[k] = r[0];
+    fw[k] = r[1];
+    gu[k][0] = r[2];
+    gu[k][1] = r[3];
+    gw[k][0] = r[4];
+    gw[k][1] = r[5];
+  }
+  const double g[3] = {f.ct_g[(int64_t)found * 3], f.ct_g[(int64_t)found * 3 + 1], f.ct_g[(int64_t)found * 3 + 2]};
+  out[0] = ct_patch(P, fu, gu, g, c);
+  out[1] = ct_patch(P, fw, gw, g, c);
 }
 
 // squared distance from t to the interval [lo, hi]
@@ -211,8 +305,14 @@ __device__ __forceinline__ void sample_scatter_nearest(const FieldDev &f, const 
 
 template <int DIM, int INTERP, typename T>
 __device__ __forceinline__ void sample_scatter(const FieldDev &f, const char *slot, const double *x, double *vel) {
-  if constexpr (INTERP == PYFTLE_INTERP_SCATTER_LINEAR) sample_scatter_linear<DIM, T>(f, slot, x, vel);
-  else sample_scatter_nearest<DIM, T>(f, slot, x, vel);
+  if constexpr (INTERP == PYFTLE_INTERP_SCATTER_LINEAR) {
+    sample_scatter_linear<DIM, T>(f, slot, x, vel);
+  } else if constexpr (INTERP == PYFTLE_INTERP_SCATTER_CUBIC) {
+    static_assert(DIM == 2, "Clough-Tocher interpolation is 2D");
+    sample_scatter_cubic<T>(f, slot, x, vel);
+  } else {
+    sample_scatter_nearest<DIM, T>(f, slot, x, vel);
+  }
 }
 
 }  // namespace pyftle

@@ -288,6 +288,8 @@ class DeviceField:
         self.dt_snap = float(dt_snap)
         dim = geom.dim
         comps = (2 if dim == 2 else 4) * (2 if layout == N.LAYOUT_PAIR else 1)
+        if self.interp == N.INTERP_SCATTER_CUBIC:
+            comps = 8  # value and gradient of u and v, padded
         esize = 4 if self.dtype == torch.float32 else 8
         # a periodic axis stores one ghost node (a copy of node 0) at index n; the cubic
         # mode stores spline coefficients -1..n per axis
@@ -344,6 +346,20 @@ class DeviceField:
 
     def slot_of(self, k: int) -> int:
         return k % self.ring_slots
+
+    def write_records(self, k: int, records: np.ndarray) -> None:
+        """Copy ready-made records ``(nodes, comps)`` of the field dtype into slot ``k``
+        (the scattered cubic mode's values and gradients)."""
+        rec = np.ascontiguousarray(records, dtype=np.float32 if self.dtype == torch.float32 else np.float64)
+        nbytes = rec.nbytes
+        if nbytes > self.slot_stride:
+            raise ValueError(f"{nbytes} bytes of records exceed the slot ({self.slot_stride} bytes)")
+        off = self._offset + self.slot_of(k) * self.slot_stride
+        dst = self._buf.narrow(0, off, nbytes)
+        src = to_device(rec.reshape(-1).view(np.uint8))
+        dst.copy_(src, non_blocking=True)
+        self._keepalive = src
+        self.resident[self.slot_of(k)] = k
 
     def pack(self, k: int, velocities, strides=None, stream=None) -> None:
         """Copy snapshot ``k`` (component-planar, reference layout) into its slot.

@@ -16,11 +16,10 @@ Differences from the reference, by design:
 * ``cubic``, ``linear`` and ``nearest`` with a ``grid_shape`` are the SciPy
   ``RegularGridInterpolator`` strategies (``GridInterpolator``,
   interpolate.py:546-762), run by the same device kernels in their own
-  interpolation modes.  Without a grid shape, ``linear`` and ``nearest`` are
-  SciPy's scattered-data strategies (``LinearInterpolator``: Delaunay +
-  barycentric; ``NearestNeighborInterpolator``), also on the device
-  (interpolate.py:188-294); ``cubic`` there (Clough-Tocher) is not provided and
-  the factory raises ``NotImplementedError``.
+  interpolation modes.  Without a grid shape they are SciPy's scattered-data
+  strategies, also on the device (interpolate.py:114-294):
+  ``CubicInterpolator`` (Clough-Tocher, 2D), ``LinearInterpolator`` (Delaunay +
+  barycentric), ``NearestNeighborInterpolator``.
 """
 
 from __future__ import annotations
@@ -274,7 +273,8 @@ class _ScatteredInterpolator(HighPerfInterpolator):
         if key != self._points_key:
             from .scatter import ScatterData
 
-            self._scatter = ScatterData(pts, need_simplices=self._mode == N.INTERP_SCATTER_LINEAR)
+            self._scatter = ScatterData(pts, need_simplices=self._mode != N.INTERP_SCATTER_NEAREST,
+                                        cubic=self._mode == N.INTERP_SCATTER_CUBIC)
             self._points_key = key
             self.dim = pts.shape[0]
             self.geom = GridGeometry.scattered(self.dim, pts.shape[1])
@@ -295,7 +295,43 @@ class _ScatteredInterpolator(HighPerfInterpolator):
         if tuple(np.shape(velocities)) != (self.dim, n):
             raise ValueError(f"velocities must have shape {(self.dim, n)}, got {tuple(np.shape(velocities))}")
         self.current += 1
-        self.field.pack(self.current, velocities)
+        self._pack(self.current, velocities)
+
+    def _pack(self, k, velocities) -> None:
+        self.field.pack(k, velocities)
+
+
+class CubicInterpolator(_ScatteredInterpolator):
+    """``CloughTocher2DInterpolator`` (interpolate.py:114-185), 2D only: C1 piecewise
+    cubic Clough-Tocher patches on the Delaunay triangulation, NaN outside the hull.
+    The vertex gradients are SciPy's global curvature-minimising estimate
+    (``CloughTocher2DInterpolator(tri, u + 1j v).grad``: the reference's own call,
+    one host call per snapshot); locating the triangle and evaluating the patch run
+    on the GPU."""
+
+    _mode = N.INTERP_SCATTER_CUBIC
+
+    def _check_dim(self, dim: int) -> None:
+        if dim == 3:
+            raise ValueError("cubic interpolator is only valid for 2D cases")
+
+    def slot_bytes(self) -> int:
+        if self.geom is None:
+            return 1
+        return self.geom.n[0] * 8 * (4 if self.dtype == torch.float32 else 8)
+
+    def _pack(self, k, velocities) -> None:
+        from scipy.interpolate import CloughTocher2DInterpolator
+
+        vel = velocities.detach().cpu().numpy() if isinstance(velocities, torch.Tensor) else np.asarray(velocities)
+        vel = np.asarray(vel, dtype=np.float64)
+        ct = CloughTocher2DInterpolator(self._scatter.tri, vel[0] + 1j * vel[1])
+        grad = np.asarray(ct.grad).reshape(vel.shape[1], 2)
+        rec = np.zeros((vel.shape[1], 8))
+        rec[:, 0], rec[:, 1] = vel[0], vel[1]
+        rec[:, 2:4] = grad.real
+        rec[:, 4:6] = grad.imag
+        self.field.write_records(k, rec)
 
 
 class LinearInterpolator(_ScatteredInterpolator):
@@ -393,7 +429,4 @@ def create_interpolator(interpolation_type: str, grid_shape=None, velocity_fn=No
         return LinearInterpolator(**kwargs)
     if interpolation_type == "nearest":
         return NearestNeighborInterpolator(**kwargs)
-    raise NotImplementedError(
-        "interpolator 'cubic' without grid_shape (SciPy Clough-Tocher on scattered points) has no "
-        "device implementation; pass grid_shape for gridded data"
-    )
+    return CubicInterpolator(**kwargs)

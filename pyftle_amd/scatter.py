@@ -41,6 +41,25 @@ def _csr(bucket, ids, n_buckets):
     return offsets.astype(np.int32), ids[order].astype(np.int32)
 
 
+def ct_factors(P, simplices, transform, neighbors):
+    """Cross-edge direction factors of the Clough-Tocher patches, (S, 3): for the edge
+    opposite vertex k, from the barycentric coordinates c of the neighbouring
+    triangle's centroid, g = (2 c_{k+2} + c_{k+1} - 1) / (2 - 3 c_{k+2} - 3 c_{k+1})
+    (indices mod 3), and -1/2 where the edge lies on the hull."""
+    S = simplices.shape[0]
+    g = np.full((S, 3), -0.5)
+    for k in range(3):
+        nb = neighbors[:, k]
+        has = nb >= 0
+        y = P[simplices[nb[has]]].mean(axis=1)  # neighbour centroids
+        T = transform[has]
+        cc = np.einsum("sij,sj->si", T[:, :2, :], y - T[:, 2, :])
+        c = np.column_stack([cc, 1.0 - cc.sum(axis=1)])
+        a, b = c[:, (k + 2) % 3], c[:, (k + 1) % 3]
+        g[has, k] = (2.0 * a + b - 1.0) / (2.0 - 3.0 * a - 3.0 * b)
+    return g
+
+
 def build_host(points, need_simplices: bool = True) -> dict:
     """The host structures of a point set ``(dim, N)`` as NumPy arrays: points
     ``(N, dim)``; point bucket grid (``p_org``, ``p_dims``, ``p_size``) with CSR
@@ -67,6 +86,7 @@ def build_host(points, need_simplices: bool = True) -> dict:
         tri = Delaunay(P)  # LinearNDInterpolator's own call, default Qhull options
         simp = np.ascontiguousarray(tri.simplices, dtype=np.int32)
         S = simp.shape[0]
+        out["tri"] = tri
         out["simplices"] = simp
         out["transform"] = np.ascontiguousarray(tri.transform, dtype=np.float64)
         out["hull"] = np.ascontiguousarray((tri.neighbors < 0).any(axis=1).astype(np.int32))
@@ -93,11 +113,14 @@ def build_host(points, need_simplices: bool = True) -> dict:
 
 
 class ScatterData:
-    """Triangulation, hull flags and bucket grids of one point set, on the device."""
+    """Triangulation, hull flags and bucket grids of one point set, on the device
+    (``cubic``: also the Clough-Tocher cross-edge factors; ``tri`` keeps SciPy's
+    triangulation on the host for the gradient estimate)."""
 
-    def __init__(self, points, need_simplices: bool = True):
-        h = build_host(points, need_simplices)
+    def __init__(self, points, need_simplices: bool = True, cubic: bool = False):
+        h = build_host(points, need_simplices or cubic)
         self.dim, self.n_points = h["dim"], h["n_points"]
+        self.tri = h.get("tri")
         self._keep = []
         d = N.Scatter()
         d.n_points = self.n_points
@@ -109,7 +132,10 @@ class ScatterData:
             d.point_origin[a] = float(h["p_org"][a])
             d.point_size[a] = float(h["p_size"][a])
         self.n_simplices = 0
-        if need_simplices:
+        if cubic:
+            tri = h["tri"]
+            d.ct_g = self._upload(ct_factors(h["points"], tri.simplices, tri.transform, tri.neighbors))
+        if need_simplices or cubic:
             self.n_simplices = d.n_simplices = h["simplices"].shape[0]
             d.simplices = self._upload(h["simplices"])
             d.transform = self._upload(h["transform"])

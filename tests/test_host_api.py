@@ -52,9 +52,12 @@ def test_interpolator_factory():
         GridInterpolator((8, 8), "quintic")
     with pytest.raises(ValueError, match="at least  4 points"):
         GridInterpolator((8, 3), "cubic")
-    # without one they are the scattered-data strategies: not provided
-    with pytest.raises(NotImplementedError):
-        create_interpolator("cubic")
+    # without one they are the scattered-data strategies (interpolate.py:114-294)
+    from pyftle_amd.interpolate import CubicInterpolator, LinearInterpolator, NearestNeighborInterpolator
+
+    assert isinstance(create_interpolator("cubic"), CubicInterpolator)
+    assert isinstance(create_interpolator("LINEAR"), LinearInterpolator)
+    assert isinstance(create_interpolator("nearest"), NearestNeighborInterpolator)
     with pytest.raises(ValueError, match="Unsupported grid dimensionality"):
         HighPerfInterpolator((8,))
 
@@ -412,3 +415,54 @@ def test_scatter_host_structures(dim):
         assert si in h["s_ids"][h["s_off"][b]:h["s_off"][b + 1]]
     # hull flags: simplices with a neighbour missing
     np.testing.assert_array_equal(h["hull"], (tri.neighbors < 0).any(axis=1))
+
+
+def test_clough_tocher_factors_and_patch_restatement():
+    """The Clough-Tocher construction the device evaluates (Bezier ordinates from vertex
+    values and gradients, cross-edge derivative toward the neighbour's centroid, -1/2 on
+    the hull; Farin 1986), restated here in NumPy from the factors of ``ct_factors``,
+    reproduces SciPy's CloughTocher2DInterpolator with SciPy's own gradients."""
+    from scipy.interpolate import CloughTocher2DInterpolator
+    from scipy.spatial import Delaunay
+
+    from pyftle_amd.scatter import ct_factors
+
+    rng = np.random.default_rng(0)
+    P = rng.uniform(0, 1, (80, 2))
+    f = np.sin(3 * P[:, 0]) + np.cos(2 * P[:, 1]) + P[:, 0] * P[:, 1]
+    tri = Delaunay(P)
+    ct = CloughTocher2DInterpolator(tri, f)
+    G = np.asarray(ct.grad).reshape(-1, 2)
+    g = ct_factors(P, tri.simplices, tri.transform, tri.neighbors)
+    xs = rng.uniform(0.1, 0.9, (300, 2))
+    ss = tri.find_simplex(xs)
+    worst = 0.0
+    for x, s_ in zip(xs[ss >= 0], ss[ss >= 0]):
+        v = tri.simplices[s_]
+        (P1, P2, P3), (f1, f2, f3), (g1, g2, g3) = P[v], f[v], G[v]
+        e12, e13, e23 = P2 - P1, P3 - P1, P3 - P2
+        c3000, c0300, c0030 = f1, f2, f3
+        c2100, c2010 = (g1 @ e12 + 3 * c3000) / 3, (g1 @ e13 + 3 * c3000) / 3
+        c1200, c0210 = (-(g2 @ e12) + 3 * c0300) / 3, (g2 @ e23 + 3 * c0300) / 3
+        c1020, c0120 = (-(g3 @ e13) + 3 * c0030) / 3, (-(g3 @ e23) + 3 * c0030) / 3
+        c2001, c0201, c0021 = (c2100 + c2010 + c3000) / 3, (c1200 + c0300 + c0210) / 3, (c1020 + c0120 + c0030) / 3
+        ga = g[s_]
+        c0111 = (ga[0] * (-c0300 + 3 * c0210 - 3 * c0120 + c0030) + (-c0300 + 2 * c0210 - c0120 + c0021 + c0201)) / 2
+        c1011 = (ga[1] * (-c0030 + 3 * c1020 - 3 * c2010 + c3000) + (-c0030 + 2 * c1020 - c2010 + c2001 + c0021)) / 2
+        c1101 = (ga[2] * (-c3000 + 3 * c2100 - 3 * c1200 + c0300) + (-c3000 + 2 * c2100 - c1200 + c2001 + c0201)) / 2
+        c1002, c0102, c0012 = (c1101 + c1011 + c2001) / 3, (c1101 + c0111 + c0201) / 3, (c1011 + c0111 + c0021) / 3
+        c0003 = (c1002 + c0102 + c0012) / 3
+        T = tri.transform[s_]
+        cc = T[:2] @ (x - T[2])
+        b = np.array([cc[0], cc[1], 1 - cc.sum()])
+        mn = b.min()
+        b1, b2, b3 = b - mn
+        b4 = 3 * mn
+        val = (b1**3 * c3000 + 3 * b1**2 * b2 * c2100 + 3 * b1**2 * b3 * c2010 + 3 * b1**2 * b4 * c2001
+               + 3 * b1 * b2**2 * c1200 + 6 * b1 * b2 * b4 * c1101 + 3 * b1 * b3**2 * c1020
+               + 6 * b1 * b3 * b4 * c1011 + 3 * b1 * b4**2 * c1002 + b2**3 * c0300 + 3 * b2**2 * b3 * c0210
+               + 3 * b2**2 * b4 * c0201 + 3 * b2 * b3**2 * c0120 + 6 * b2 * b3 * b4 * c0111
+               + 3 * b2 * b4**2 * c0102 + b3**3 * c0030 + 3 * b3**2 * b4 * c0021 + 3 * b3 * b4**2 * c0012
+               + b4**3 * c0003)
+        worst = max(worst, abs(val - ct(x[None])[0]))
+    assert worst < 1e-13
