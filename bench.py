@@ -200,17 +200,18 @@ def build_batch64(torch, h=0.05, n_fields=64):
                         "seed_steps_total": seed_steps})
 
 
-def build_surrogate2d(torch, nsx=2048, nsy=1024, n_t0=32, members=4, epochs=200, n_traj=4_000_000):
+def build_surrogate2d(torch, nsx=2048, nsy=1024, n_t0=32, members=4, epochs=200, n_traj=4_000_000, depth=5):
     """Double-gyre flow-map surrogate: 512x256 snapshots over t in [0, 20] (the flow is
     10-periodic, windows start in [0, 10)), RK4 h=0.05 over T=10."""
     from pyftle_amd import surrogate_ftle as SF
 
     field = _gyre_field_on_device(torch, 512, 256, 201, 0.25, 2.0 * math.pi / 10.0)
     return dict(name="surrogate2d", field=field, nsx=nsx, nsy=nsy, n_t0=n_t0, members=members,
-                epochs=epochs, n_traj=n_traj, T=10.0, h=0.05, dim=2, SF=SF,
+                epochs=epochs, n_traj=n_traj, T=10.0, h=0.05, dim=2, SF=SF, depth=depth,
                 config={"workload": "surrogate2d", "flow": "double gyre A=0.1 eps=0.25 omega=2pi/10",
                         "grid": [512, 256], "snapshots": 201, "seeds": [nsx, nsy], "t0_values": n_t0,
-                        "T": 10.0, "rk4_h": 0.05, "mlp": "64 Fourier freqs on (x,y,t0), 5x256 SiLU -> 2",
+                        "T": 10.0, "rk4_h": 0.05,
+                        "mlp": f"64 Fourier freqs on (x,y,t0), {depth}x256 SiLU -> 2",
                         "members": members, "epochs": epochs, "trajectories": n_traj,
                         "fallback_tol": 1e-3, "dtype": "bf16 MFMA, fp32 accumulate"})
 
@@ -227,7 +228,7 @@ def run_surrogate(torch, wl, args):
                                  n_bins=500)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t
-    spec = S.MLPSpec(sigma=1.0)
+    spec = S.MLPSpec(sigma=1.0, depth=wl.get("depth", 5))
     t = time.perf_counter()
     ens = SF.fit_ensemble(X, D, spec, members=wl["members"], epochs=wl["epochs"], batch=16384, lr=3e-3, log=log)
     torch.cuda.synchronize()
@@ -865,6 +866,9 @@ def main():
                     help="surrogate2d: ensemble spread above which a seed falls back to RK4")
     ap.add_argument("--surrogate-epochs", type=int, default=0,
                     help="surrogate2d / surrogate3d: training epochs (0 = the workload's default)")
+    ap.add_argument("--surrogate-depth", type=int, default=0,
+                    help="surrogate2d: hidden layers of width 256 (0 = the 5 of the specified network; "
+                         "other depths are an accuracy/speed exploration)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy check")
     ap.add_argument("--groups-per-block", type=int, default=0)
@@ -898,10 +902,12 @@ def main():
     _native.load()
     torch.cuda.set_device(0)
     log(f"building workload {args.workload}")
+    kw = {}
     if args.workload in ("surrogate2d", "surrogate3d") and args.surrogate_epochs > 0:
-        wl = WORKLOADS[args.workload](torch, epochs=args.surrogate_epochs)
-    else:
-        wl = WORKLOADS[args.workload](torch)
+        kw["epochs"] = args.surrogate_epochs
+    if args.workload == "surrogate2d" and args.surrogate_depth > 0:
+        kw["depth"] = args.surrogate_depth
+    wl = WORKLOADS[args.workload](torch, **kw)
     if wl["name"] == "batch64":
         return run_batch(torch, wl, args)
     if wl["name"] == "surrogate2d":
