@@ -871,6 +871,9 @@ def main():
                          "other depths are an accuracy/speed exploration)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy check")
+    ap.add_argument("--seed-order", default="auto", choices=["auto", "morton", "none"],
+                    help="abc128 / gyre512: process seed groups in Morton order of their cells (results are "
+                         "identical; computed inside every timed step); auto = from 2^18 seed groups")
     ap.add_argument("--groups-per-block", type=int, default=0)
     ap.add_argument("--lane-order", type=int, default=-1)
     ap.add_argument("--xcd-remap", type=int, default=-1)
@@ -924,21 +927,29 @@ def main():
     if args.xcd_remap >= 0:
         cfg.xcd_remap = args.xcd_remap
     ftle = torch.empty(wl["n_seeds"], dtype=torch.float64, device="cuda")
+    nb = 2 * wl["dim"]
 
-    def launch():
+    use_order = args.seed_order == "morton" or (args.seed_order == "auto" and wl["n_seeds"] >= 1 << 18)
+
+    def order():
+        # part of every step (inside the wall-clock time, outside the kernel's events)
+        return W.seed_order(wl["field"], wl["pos"], nb) if use_order else None
+
+    def launch(o):
         W.advect_ftle(wl["field"], wl["pos"], method="rk4", time_mode=wl["time_mode"],
                       n_steps=wl["steps"], h=wl["h"], t_start=wl["t_start"],
-                      map_period=wl["map_period"], ftle_out=ftle, cfg=cfg)
+                      map_period=wl["map_period"], ftle_out=ftle, cfg=cfg, order=o)
 
     for _ in range(args.warmup):
-        launch()
+        launch(order())
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()  # the stream the kernels are enqueued on
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     t0 = time.perf_counter()
     for a, b in evs:
+        o = order()
         a.record(stream)
-        launch()
+        launch(o)
         b.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -978,7 +989,7 @@ def main():
         "seed_steps_per_s": seed_steps * args.steps / wall,
         "roofline": roof,
         "launch": {"groups_per_block": cfg.groups_per_block, "lane_order": cfg.lane_order,
-                   "xcd_remap": cfg.xcd_remap},
+                   "xcd_remap": cfg.xcd_remap, "seed_order": "morton" if use_order else "none"},
     }
     if not args.no_accuracy:
         log("accuracy against FP64")

@@ -157,6 +157,12 @@ typedef struct pyftle_advect_opts {
   int32_t groups_per_block; /* seeds per work-group (grouped kernels), 0 = default */
   int32_t xcd_remap;      /* 1: map consecutive work to one XCD's L2 (8 XCDs) */
   int32_t compute;        /* PYFTLE_COMPUTE_* */
+  int32_t reserved;       /* 0 */
+  const int32_t *seed_order; /* device (n_groups,) permutation or NULL: the k-th seed group
+                                processed is group seed_order[k] (positions read and results
+                                written at its own index, so results do not depend on it; an
+                                entry outside [0, n_groups) processes group k instead);
+                                see pyftle_seed_morton_codes */
 } pyftle_advect_opts;
 
 /* ---- library ---- */
@@ -205,6 +211,15 @@ int pyftle_field_sample(const pyftle_field *field, int32_t time_mode, int32_t sn
  * the field's dtype; FP32 keeps a compensated (hi, lo) position. */
 int pyftle_advect(const pyftle_field *field, const pyftle_advect_opts *opts, void *positions,
                   int32_t pos_dtype, int64_t n_particles, void *vprev, void *stream);
+
+/* ---- seed order (no reference counterpart): Morton (Z-order) code of the cell
+ * floor((centroid - lo) * inv_cell) (clamped to [0, 2^20) per axis) of each seed group of a
+ * block-ordered (nb*n_groups, dim) position array; codes (n_groups,) in int64 storage.
+ * Sorting the groups by code (stable) and passing the permutation as opts->seed_order
+ * puts spatially close seeds in the same waves, so a wave's gathers touch fewer lines. */
+int pyftle_seed_morton_codes(const void *positions, int32_t pos_dtype, int64_t n_groups, int32_t nb,
+                             int32_t dim, const double lo[3], const double inv_cell[3], int64_t *codes,
+                             void *stream);
 
 /* ---- fused window: FTLESolver.run + _compute_ftle (ftle_solver.py:68-161) ----
  * positions_in: block-ordered (nb*n_groups, dim) [L,R,T,B(,F,Bk)] of pos_dtype

@@ -87,6 +87,8 @@ class AdvectOpts(ctypes.Structure):
         ("groups_per_block", ctypes.c_int32),
         ("xcd_remap", ctypes.c_int32),
         ("compute", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("seed_order", ctypes.c_void_p),
     ]
 
 
@@ -109,6 +111,8 @@ SIGNATURES = {
     "pyftle_pack_spline": [_FP, _i32, _vp, _i32, ctypes.POINTER(ctypes.c_int64), _vp, _vp],
     "pyftle_field_sample": [_FP, _i32, _i32, _f64, _vp, _i32, _i64, _vp, _vp],
     "pyftle_advect": [_FP, _OP, _vp, _i32, _i64, _vp, _vp],
+    "pyftle_seed_morton_codes": [_vp, _i32, _i64, _i32, _i32, ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_double), _vp, _vp],
     "pyftle_advect_ftle": [_FP, _OP, _vp, _i32, _i64, _vp, _f64, _vp, _vp],
     "pyftle_flow_map_jacobian": [_vp, _vp, _i32, _i32, _i64, _vp, _vp],
     "pyftle_cauchy_green": [_vp, _i32, _i64, _vp, _vp],

@@ -51,11 +51,53 @@ static void dispatch(const pyftle_field *field, const AdvectArgs &a, const Launc
   else launch_advect_3d(a, sp);
 }
 
+// Morton code of the cell of each seed group's centroid (20 bits per axis).
+__device__ __forceinline__ uint64_t spread_bits(uint64_t v, int dim) {
+  uint64_t r = 0;
+  for (int b = 0; b < 20; ++b) r |= ((v >> b) & 1ull) << (b * dim);
+  return r;
+}
+
+__global__ void __launch_bounds__(256) seed_morton_kernel(const void *pos, int pos_f64, int64_t n, int nb,
+                                                          int dim, double lo0, double lo1, double lo2,
+                                                          double ic0, double ic1, double ic2, int64_t *codes) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const double lo[3] = {lo0, lo1, lo2}, ic[3] = {ic0, ic1, ic2};
+  uint64_t code = 0;
+  for (int a = 0; a < dim; ++a) {
+    double c = 0.0;
+    for (int j = 0; j < nb; ++j) {
+      const int64_t i = ((int64_t)j * n + s) * dim + a;
+      c += pos_f64 ? ((const double *)pos)[i] : (double)((const float *)pos)[i];
+    }
+    double q = (c / nb - lo[a]) * ic[a];
+    q = (q == q) ? fmin(fmax(q, 0.0), 1048575.0) : 0.0;
+    code |= spread_bits((uint64_t)q, dim) << a;
+  }
+  codes[s] = (int64_t)code;
+}
+
 }  // namespace pyftle
 
 using namespace pyftle;
 
 extern "C" {
+
+int pyftle_seed_morton_codes(const void *positions, int32_t pos_dtype, int64_t n_groups, int32_t nb,
+                             int32_t dim, const double lo[3], const double inv_cell[3], int64_t *codes,
+                             void *stream) {
+  if (pos_dtype != PYFTLE_F32 && pos_dtype != PYFTLE_F64) return fail("seed_morton_codes: bad dtype");
+  if (dim != 2 && dim != 3) return fail("seed_morton_codes: dim must be 2 or 3");
+  if (nb < 1 || n_groups < 0) return fail("seed_morton_codes: bad sizes");
+  if (n_groups == 0) return PYFTLE_OK;
+  if (!positions || !codes || !lo || !inv_cell) return fail("seed_morton_codes: NULL pointer");
+  const int64_t blocks = (n_groups + 255) / 256;
+  seed_morton_kernel<<<(unsigned)blocks, 256, 0, as_stream(stream)>>>(
+      positions, pos_dtype == PYFTLE_F64, n_groups, nb, dim, lo[0], lo[1], dim == 3 ? lo[2] : 0.0,
+      inv_cell[0], inv_cell[1], dim == 3 ? inv_cell[2] : 0.0, codes);
+  return check_launch("seed_morton_codes");
+}
 
 int pyftle_advect(const pyftle_field *field, const pyftle_advect_opts *opts, void *positions,
                   int32_t pos_dtype, int64_t n_particles, void *vprev, void *stream) {
@@ -85,6 +127,7 @@ int pyftle_advect(const pyftle_field *field, const pyftle_advect_opts *opts, voi
   a.t_start = opts->t_start;
   a.h = opts->h;
   a.map_period = 1.0;
+  a.order = opts->seed_order;
   const int64_t blocks = (n_particles + a.G - 1) / a.G;
   if (blocks > INT32_MAX) return fail("advect: too many particles");
   LaunchSpec sp{opts->method, opts->time_mode, field->layout == PYFTLE_LAYOUT_PAIR,
@@ -127,6 +170,7 @@ int pyftle_advect_ftle(const pyftle_field *field, const pyftle_advect_opts *opts
   a.t_start = opts->t_start;
   a.h = opts->h;
   a.map_period = map_period;
+  a.order = opts->seed_order;
   const int64_t blocks = (n_groups + G - 1) / G;
   if (blocks > INT32_MAX) return fail("advect_ftle: too many seeds");
   int threads = G * nb;

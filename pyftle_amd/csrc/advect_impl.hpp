@@ -46,7 +46,15 @@ struct AdvectArgs {
   double t_start;
   double h;
   double map_period;
+  const int *order;  // processing order of the seed groups (NULL: identity)
 };
+
+// index of the k-th processed seed group (an entry outside [0, n_groups) maps to k)
+__device__ __forceinline__ int64_t seed_at(const AdvectArgs &a, int64_t k) {
+  if (!a.order) return k;
+  const int64_t q = a.order[k];
+  return (q >= 0 && q < a.n_groups) ? q : k;
+}
 
 template <int PREC, bool STORE64>
 struct PrecTraits;
@@ -177,8 +185,9 @@ __device__ __forceinline__ void ftle_epilogue(const AdvectArgs &a, int64_t lb, i
   }
   __syncthreads();
   if (t < G) {
-    const int64_t seed = lb * G + t;
-    if (seed < a.n_groups) {
+    const int64_t k = lb * G + t;
+    if (k < a.n_groups) {
+      const int64_t seed = seed_at(a, k);
       double F[DIM * DIM];
 #pragma unroll
       for (int c = 0; c < DIM; ++c) {
@@ -212,7 +221,7 @@ __global__ void __launch_bounds__(512) advect_kernel(AdvectArgs a) {
   }
   const int64_t s = lb * G + sl;
   const bool active = (t < G * nb) && (s < a.n_groups);
-  const int64_t pidx = (int64_t)j * a.n_groups + s;
+  const int64_t pidx = (int64_t)j * a.n_groups + (active ? seed_at(a, s) : s);
 
   double x0[DIM];
   PosState<Pos> X[DIM];
@@ -460,7 +469,7 @@ __device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t l
   }
   const int64_t s = lb * G + sl;
   const bool active = (t < G * nb) && (s < a.n_groups);
-  const int64_t pidx = (int64_t)j * a.n_groups + s;
+  const int64_t pidx = (int64_t)j * a.n_groups + (active ? seed_at(a, s) : s);
   const FieldDev &f = a.f;
 
   double x0[DIM];

@@ -9,7 +9,8 @@ keeps the reference signature and result.  ``run()``:
 3. as soon as the ring is full (or the window is complete) the advection
    kernel takes all pending steps in one launch, each step reading its own
    snapshot frozen for every stage (integrate.py:323-339); the window's last
-   launch also runs the FP64 FTLE epilogue with
+   launch also runs the FP64 FTLE epilogue (large single-launch windows process their
+   seed groups in Morton order of their cells, which leaves results unchanged) with
    ``map_period = (num_steps - 1) * |dt|`` (ftle_solver.py:156,160);
 4. progress messages ``(id, i)`` and ``(id, "done")`` go to the queue
    (ftle_solver.py:104-110); the field is written or returned (112-120).
@@ -80,9 +81,11 @@ class FTLESolver:
                 field = interp.field
                 last = stop == num_steps
                 if start == 0 and last:
+                    n_groups = init.shape[0] // (2 * init.shape[1])
                     ftle = W.advect_ftle(field, init, method=method, time_mode="frozen",
                                          n_steps=num_steps, h=timestep, map_period=map_period,
-                                         first=0, positions_out=dev, cfg=self.launch)
+                                         first=0, positions_out=dev, cfg=self.launch,
+                                         order="morton" if n_groups >= 1 << 18 else None)
                 else:
                     if method == "ab2" and vprev is None:
                         vprev = torch.zeros_like(dev)

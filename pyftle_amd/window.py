@@ -36,6 +36,29 @@ class LaunchConfig:
     compute: int = N.COMPUTE_NATIVE
 
 
+def seed_order(field: DeviceField, positions: torch.Tensor, nb: int) -> torch.Tensor:
+    """Processing order of the seed groups of a block-ordered ``(nb*N, dim)`` array:
+    a stable sort by the Morton (Z-order) code of the grid cell holding each group's
+    centroid (``pyftle_seed_morton_codes``).  Spatially close seeds then share waves and
+    their gathers share cache lines; results do not depend on the order."""
+    dim = field.geom.dim
+    n = positions.shape[0] // nb
+    codes = torch.empty(n, dtype=torch.int64, device=positions.device)
+    if field.interp in (N.INTERP_SCATTER_LINEAR, N.INTERP_SCATTER_NEAREST, N.INTERP_SCATTER_CUBIC):
+        lo_t, hi_t = torch.aminmax(positions, dim=0)
+        lo = [float(v) for v in lo_t.tolist()]
+        ext = [max(float(h - l), 1e-300) for l, h in zip(lo, hi_t.tolist())]
+        inv = [1024.0 / e for e in ext]
+    else:
+        lo = list(field.geom.origin)
+        inv = [1.0 / sp for sp in field.geom.spacing]
+    lo3 = (ctypes.c_double * 3)(*(lo + [0.0] * (3 - dim)))
+    inv3 = (ctypes.c_double * 3)(*(inv + [0.0] * (3 - dim)))
+    N.call("pyftle_seed_morton_codes", ptr(positions), dtype_code(positions.dtype), n, nb, dim, lo3, inv3,
+           ptr(codes), stream_handle())
+    return torch.argsort(codes, stable=True).to(torch.int32)
+
+
 def make_opts(method, time_mode, n_steps, h, first=0, t_start=0.0, have_prev=False,
               cfg: LaunchConfig | None = None) -> N.AdvectOpts:
     cfg = cfg or LaunchConfig()
@@ -57,8 +80,11 @@ def make_opts(method, time_mode, n_steps, h, first=0, t_start=0.0, have_prev=Fal
 def advect_ftle(field: DeviceField, positions: torch.Tensor, *, method="rk4", time_mode="frozen",
                 n_steps: int, h: float, map_period: float, first: int = 0, t_start: float = 0.0,
                 positions_out: torch.Tensor | None = None, ftle_out: torch.Tensor | None = None,
-                cfg: LaunchConfig | None = None, stream=None) -> torch.Tensor:
-    """Advect block-ordered seed groups and return the FP64 FTLE per seed."""
+                cfg: LaunchConfig | None = None, stream=None, order=None) -> torch.Tensor:
+    """Advect block-ordered seed groups and return the FP64 FTLE per seed.
+
+    ``order``: ``None``, ``"morton"`` (``seed_order``) or an int32 device permutation of the
+    seed groups giving the order in which they are processed (results are the same)."""
     if not positions.is_cuda:
         raise ValueError("positions must be a device tensor")
     positions = positions.contiguous()
@@ -72,6 +98,15 @@ def advect_ftle(field: DeviceField, positions: torch.Tensor, *, method="rk4", ti
     if ftle_out is None:
         ftle_out = torch.empty(n_groups, dtype=torch.float64, device=positions.device)
     opts = make_opts(method, time_mode, n_steps, h, first, t_start, False, cfg)
+    if isinstance(order, str):
+        if order != "morton":
+            raise ValueError(f"unknown seed order {order!r}")
+        order = seed_order(field, positions, nb)
+    if order is not None:
+        if order.dtype != torch.int32 or not order.is_cuda or order.numel() != n_groups:
+            raise ValueError("order must be an int32 device permutation of the seed groups")
+        order = order.contiguous()
+        opts.seed_order = order.data_ptr()
     pout = ptr(positions_out) if positions_out is not None else ctypes.c_void_p(0)
     N.call("pyftle_advect_ftle", ctypes.byref(field.desc), ctypes.byref(opts), ptr(positions),
            dtype_code(positions.dtype), n_groups, pout, float(map_period), ptr(ftle_out),

@@ -130,3 +130,27 @@ def test_empty_and_zero_step_windows():
                          positions_out=out)
     assert torch.equal(out, x0)
     assert torch.all(ftle == 0.0)
+
+
+@pytest.mark.parametrize("precision", ["float32", "float64"])
+def test_seed_order_leaves_results_unchanged(precision):
+    """Processing the seed groups in Morton order (or any permutation) gives bit-identical
+    FTLE and final positions: every group is independent and written at its own index."""
+    from pyftle_amd import window as W
+
+    vels, pts = H.double_gyre_window(nx=64, ny=32, n_snap=6)
+    x0 = torch.as_tensor(H.seeds_2d(37, 19), device="cuda")
+    f = _field(vels, pts, (64, 32), precision)
+    kw = dict(method="rk4", time_mode="frozen", n_steps=6, h=0.1, map_period=0.5)
+    out0, out1, out2 = (torch.empty_like(x0) for _ in range(3))
+    base = W.advect_ftle(f, x0, positions_out=out0, **kw)
+    morton = W.advect_ftle(f, x0, positions_out=out1, order="morton", **kw)
+    n = x0.shape[0] // 4
+    perm = torch.randperm(n, generator=torch.Generator().manual_seed(1)).to(torch.int32).cuda()
+    rand = W.advect_ftle(f, x0, positions_out=out2, order=perm, **kw)
+    order = W.seed_order(f, x0, 4)
+    assert sorted(order.cpu().tolist()) == list(range(n))
+    assert not torch.equal(order.cpu(), torch.arange(n, dtype=torch.int32))  # it does reorder
+    for got, pos in ((morton, out1), (rand, out2)):
+        assert torch.equal(got, base)
+        assert torch.equal(pos, out0)
