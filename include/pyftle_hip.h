@@ -275,6 +275,7 @@ typedef struct pyftle_batch_field {
   double t_start;
   double h;
   double map_period;
+  const int32_t *seed_order; /* processing order of the seed groups or NULL (as in pyftle_advect_opts) */
 } pyftle_batch_field;
 
 typedef struct pyftle_batch_opts {

@@ -37,6 +37,7 @@ class BatchField(ctypes.Structure):
         ("t_start", ctypes.c_double),
         ("h", ctypes.c_double),
         ("map_period", ctypes.c_double),
+        ("seed_order", ctypes.c_void_p),
     ]
 
 
@@ -85,6 +86,7 @@ class Window:
     first_snapshot: int = 0
     positions_out: torch.Tensor | None = None
     ftle: torch.Tensor | None = dc_field(default=None)
+    order: torch.Tensor | None = None  # int32 device permutation of the seed groups (window.seed_order)
 
 
 class BatchPlan:
@@ -133,6 +135,11 @@ class BatchPlan:
             bf.t_start = float(w.t_start)
             bf.h = float(w.h)
             bf.map_period = float(w.map_period)
+            if w.order is not None:
+                if w.order.dtype != torch.int32 or not w.order.is_cuda or w.order.numel() != n:
+                    raise ValueError(f"window {i}: order must be an int32 device permutation of {n} groups")
+                w.order = w.order.contiguous()
+                bf.seed_order = w.order.data_ptr()
         o = BatchOpts()
         o.method = METHODS[method] if isinstance(method, str) else int(method)
         o.time_mode = TIME_MODES[time_mode] if isinstance(time_mode, str) else int(time_mode)

@@ -163,7 +163,7 @@ int pyftle_batch_create(const pyftle_batch_field *fields, int32_t n_fields,
     a.t_start = bf.t_start;
     a.h = bf.h;
     a.map_period = bf.map_period;
-    a.order = nullptr;
+    a.order = bf.seed_order;
     blocks[i] = (bf.n_groups + G - 1) / G;
     n_items += blocks[i];
   }
