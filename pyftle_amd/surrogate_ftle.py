@@ -115,7 +115,8 @@ class SurrogateFTLE:
         pos = neighbour_groups(seeds, self.delta)
         steps = int(round(self.T / abs(self.h)))
         return W.advect_ftle(self.field, pos, method="rk4", time_mode=self.time_mode, n_steps=steps,
-                             h=self.h, t_start=t0, map_period=steps * abs(self.h))
+                             h=self.h, t_start=t0, map_period=steps * abs(self.h),
+                             order="morton" if seeds.shape[0] >= 1 << 18 else None)
 
     def ftle(self, seeds: torch.Tensor, t0: float) -> SurrogateResult:
         f, spread = self.predict(seeds, t0)
@@ -142,8 +143,11 @@ def rk4_ftle_windows(field: DeviceField, seeds: torch.Tensor, t0s, T: float, h: 
     """Plain RK4 FTLE of the same seeds for several window starts in one batched launch."""
     steps = int(round(T / abs(h)))
     pos = neighbour_groups(seeds, delta)
-    wins = [Window(field=field, positions=pos, n_steps=steps, h=h, t_start=float(t0), map_period=steps * abs(h))
-            for t0 in t0s]
+    dim = seeds.shape[1]
+    # one Morton order of the shared seeds for every window (results do not depend on it)
+    order = W.seed_order(field, pos, 2 * dim) if seeds.shape[0] >= 1 << 18 else None
+    wins = [Window(field=field, positions=pos, n_steps=steps, h=h, t_start=float(t0), map_period=steps * abs(h),
+                   order=order) for t0 in t0s]
     plan = BatchPlan(wins, method="rk4", time_mode=time_mode, cfg=cfg)
     return plan, wins
 
