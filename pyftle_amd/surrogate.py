@@ -250,8 +250,11 @@ class FlowMapMLP:
         return ws
 
     # -- forward / JVP -----------------------------------------------------
-    def forward(self, x: torch.Tensor, n_tangents: int = 0):
-        """Outputs ``(M, d_out)`` and, with ``n_tangents``, d out / d x_a for a < n_tangents."""
+    def forward(self, x: torch.Tensor, n_tangents: int = 0, keep_z: bool = True):
+        """Outputs ``(M, d_out)`` and, with ``n_tangents``, d out / d x_a for a < n_tangents.
+
+        ``keep_z``: store every hidden layer's pre-activation (read by the tangent rows and by
+        the backward pass); primal-only inference skips it and writes half the bytes."""
         s = self.spec
         M = x.shape[0]
         Mp = pad128(max(M, 1))
@@ -266,8 +269,9 @@ class FlowMapMLP:
         h, fan = ws["F"], F2
         for l in range(s.depth):
             H = ws["H"][l]
+            zp = _p(ws["Z"][l]) if (keep_z or n_tangents) else ctypes.c_void_p(0)
             gemm(_p(h), fan, 1, _p(self.Wb(l)), fan, 1, Mp, s.width, fan, _p(H), s.width,
-                 epi=EPI_BIAS_SILU, bias=_p(self.b(l)), Z=_p(ws["Z"][l]))
+                 epi=EPI_BIAS_SILU, bias=_p(self.b(l)), Z=zp)
             if n_tangents:
                 gemm(_off(h, Mp * fan), fan, 1, _p(self.Wb(l)), fan, 1, Mp * n_tangents, s.width, fan,
                      _off(H, Mp * s.width), s.width, epi=EPI_DSILU, Z=_p(ws["Z"][l]), z_rows=Mp)
@@ -435,7 +439,7 @@ class FlowMapEnsemble:
                 outs.append(y.clone())
                 jacs.append(torch.stack([t.clone() for t in tans], dim=2))
             else:
-                outs.append(m.forward(x).clone())
+                outs.append(m.forward(x, keep_z=False).clone())
         Y = torch.stack(outs)
         Jd = torch.stack(jacs)
         mean = Y.mean(0)
