@@ -364,6 +364,18 @@ int pyftle_mlp_fused_forward(const float *x, int32_t d_in, int32_t M, const floa
                              const float *bout, int32_t O, float out_scale, int32_t n_tangents,
                              float *Y, void *stream);
 
+/* Fused inference for hidden width 512 (activations kept in LDS, weights streamed from L2
+ * into registers): 64 frequencies, depth 2..8, (outputs, tangents) in {(2, 0), (3, 0),
+ * (3, 3)}.  W_packed: every layer (512 x K) as written by pyftle_mlp_pack_wide (rows a
+ * multiple of 16, K of 32), layer 0 (K = 128) first; W_packed, bias (depth x 512) and Wout
+ * (O x 512) 16-byte aligned; Y (1 + n_tangents, M, O) with the output scale applied. */
+int pyftle_mlp_pack_wide(const float *W, int32_t rows, int32_t K, void *out, void *stream);
+int pyftle_mlp_wide_forward(const float *x, int32_t d_in, int32_t M, const float *scale,
+                            const float *Bf, int32_t n_freq, const void *W_packed,
+                            const float *bias, int32_t depth, int32_t width, const float *Wout,
+                            const float *bout, int32_t O, float out_scale, int32_t n_tangents,
+                            float *Y, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

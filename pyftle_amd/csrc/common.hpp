@@ -24,6 +24,19 @@ int check_launch(const char *what);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// compute units of the current device (cached per device; 256 if the query fails)
+inline int device_cu_count() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
+
 // ---------------------------------------------------------------------------
 // device view of a pyftle_field (passed by value as a kernel argument)
 // ---------------------------------------------------------------------------

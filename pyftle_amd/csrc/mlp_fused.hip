@@ -430,24 +430,12 @@ __global__ void pack_fused_kernel(const float *W, int rows, int K, int permute, 
 
 static_assert(kThreads == kNFreq * kMaxIn && kThreads == kWidth, "constant staging maps one thread per entry");
 
-int cu_count() {
-  static int cus[64] = {0};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (!cus[dev]) {
-    int n = 0;
-    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) n = 256;
-    cus[dev] = n;
-  }
-  return cus[dev];
-}
-
 template <int G, int NT, int O>
 void launch_fused(const FusedArgs &a, hipStream_t st) {
   // one work-group per CU (the LDS weight buffers admit one); each loops over seed tiles
   const int64_t tile_seeds = (int64_t)kWaves * G * kSeedsPerWave;
   const int64_t tiles = ((int64_t)a.M + tile_seeds - 1) / tile_seeds;
-  const unsigned grid = (unsigned)std::min<int64_t>(tiles, cu_count());
+  const unsigned grid = (unsigned)std::min<int64_t>(tiles, device_cu_count());
   fused_mlp_kernel<G, NT, O><<<grid, kThreads, 0, st>>>(a);
 }
 

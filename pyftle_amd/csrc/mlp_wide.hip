@@ -1,0 +1,412 @@
+// Fused surrogate inference for hidden width 512: Fourier features -> depth x (512, SiLU)
+// -> O outputs, with forward-mode tangents, in one kernel.
+//
+// No reference counterpart: the learned flow-map approximation of the RK4 path
+// (integrate.py:265-342).  At width 512 a seed's activations and tangent rows do not fit
+// in a wave's registers (the width-256 kernel in mlp_fused.hip keeps them there), and the
+// layer-by-layer kernels move 2 KB per row and layer through HBM, which at K = 512 is
+// below the MFMA ridge.  Here a work-group keeps the activations of its 64 rows in LDS for
+// the whole network, and only the weights move: from L2 straight into registers.
+//
+// Formulation: out^T = W in^T per layer with v_mfma_f32_16x16x32_bf16.  The weights are
+// the A operand (16 output features x 32 inputs per fragment), the activations the B
+// operand (32 inputs x 16 rows) read from LDS.  Wave w owns output features
+// 128 w .. 128 w + 127 (8 tiles of 16) for all 64 rows of the work-group (4 row tiles of
+// 16), so each weight fragment feeds 4 MFMAs and each activation fragment 8.  The weights
+// are packed fragment by fragment (pyftle_mlp_pack_wide): the 64 lanes' 16-byte pieces of
+// one fragment are 1 KB contiguous, one coalesced load per wave.
+//
+// Rows of a tile: S seeds x (1 + NT) row blocks of 16; row tile q (1 + NT) + a holds seeds
+// 16 q .. 16 q + 15 of the tile, a = 0 the primal rows, a > 0 the derivative along input
+// a - 1.  A lane holds the same (seed, feature) pair for the primal and its tangent rows,
+// so SiLU'(z) of the tangent rows uses the primal pre-activation from registers, in FP32.
+//
+// Per layer: the MFMAs over K (the weight fragments of the next k-step in flight; the
+// next layer's first ones are issued during the last k-step), a barrier, the epilogue
+// SiLU(z + b) written back to LDS in place as bf16, a barrier.  Two work-groups share a CU
+// (70 KB of LDS each), so one's epilogue runs under the other's MFMAs.  The output layer
+// (O <= 3) is FP32 on the last hidden layer's registers, reduced over lanes and waves.
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace pyftle {
+namespace {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kWidth = 512;                         // hidden width
+constexpr int kFeat = 128;                          // Fourier features: 64 frequencies, sin and cos
+constexpr int kNFreq = kFeat / 2;
+constexpr int kWaves = 4;
+constexpr int kThreads = 64 * kWaves;
+constexpr int kFragTiles = kWidth / 16;             // 16-feature tiles of a layer
+constexpr int kTilesPerWave = kFragTiles / kWaves;  // 8
+constexpr int kRowTiles = 4;                        // 16-row tiles of a work-group
+constexpr int kRows = 16 * kRowTiles;
+constexpr int kMaxDepth = 8;
+constexpr int kMaxIn = 4;
+
+struct WideArgs {
+  const float *x;      // (M, d_in) inputs
+  const float *scale;  // (d_in) input scaling before the Fourier projection
+  const float *Bf;     // (64, d_in) frequencies, in revolutions per unit
+  const __bf16 *W;     // packed hidden weights (pyftle_mlp_pack_wide order), layer after layer
+  int w_bytes;         // their size
+  const float *bias;   // (depth, 512)
+  const float *Wout;   // (O, 512)
+  const float *bout;   // (O)
+  float *Y;            // (1 + n_tangents, M, O)
+  float out_scale;
+  int d_in;
+  int M;
+  int depth;
+};
+
+// Byte offset of layer L in the packed weights.
+__device__ __forceinline__ int layer_bytes(int L) {
+  return L == 0 ? 0 : 2 * (kWidth * kFeat + (L - 1) * kWidth * kWidth);
+}
+
+// This lane's 8 values of a fragment of the packed weights: buffer load at the wave-uniform
+// byte offset `sofs` (layer, k-step and feature tile, in a scalar register) plus the lane's
+// 16-byte piece `vofs`, so no per-load 64-bit address is formed.
+__device__ __forceinline__ bf16x8_t load_frag(__amdgpu_buffer_rsrc_t w, int vofs, int sofs) {
+  return __builtin_bit_cast(bf16x8_t, __builtin_amdgcn_raw_buffer_load_b128(w, vofs, sofs, 0));
+}
+
+// Byte offset of fragment (k-step ks, feature tile ft) inside a packed layer.
+constexpr int frag_bytes(int ks, int ft) { return (ks * kFragTiles + ft) * 1024; }
+
+// LDS activations: row r holds 512 bf16 (64 units of 16 bytes); unit u of row r is stored
+// at unit u ^ (r % 16), so the 16 rows read by one B fragment land in distinct banks.
+__device__ __forceinline__ int act_off(int r, int u) { return r * kWidth + ((u ^ (r & 15)) << 3); }
+
+// The lane index as a value the compiler must take as new at each call: the per-lane
+// offsets derived from it are then recomputed where they are used (a few VALU
+// instructions) instead of being hoisted out of the tile and layer loops, where dozens of
+// them would stay live next to the accumulators and spill.  The statement emits no
+// instruction.
+__device__ __forceinline__ int opaque_lane(int lane) {
+  asm volatile("" : "+v"(lane));
+  return lane;
+}
+
+// 1 / (1 + e^-z) with the hardware exp2 and reciprocal (about 1 ulp each; the results are
+// rounded to bf16 or summed in FP32 afterwards).  e^-z overflows to +inf for z < -88 and the
+// reciprocal then gives 0, the correct limit.
+__device__ __forceinline__ float sigmoid_f(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * -1.4426950408889634f));
+}
+
+// acc[i][t] = (output features of this wave's tile i) x (rows of row tile t), over the K
+// inputs held in LDS.  On entry ring[i] holds tile i of k-step 0 of this layer; as soon as
+// the 4 MFMAs of a tile have been issued its slot is refilled with the same tile of the
+// next k-step (or of k-step 0 of the layer at `next`), so each load has a whole k-step
+// (32 MFMAs of this wave, interleaved with the other work-group's wave on the SIMD) to land.
+template <int K>
+__device__ __forceinline__ void layer_mma(__amdgpu_buffer_rsrc_t w, int lbytes, int next_lbytes,
+                                          const __bf16 *act, bf16x8_t (&ring)[kTilesPerWave],
+                                          f32x4_t (&acc)[kTilesPerWave][kRowTiles], int ft0, int lane_in) {
+  constexpr int KS = K / 32;
+  const int lane = opaque_lane(lane_in);
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < kTilesPerWave; ++i)
+#pragma unroll
+    for (int t = 0; t < kRowTiles; ++t) acc[i][t] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+  bf16x8_t bfr[2][kRowTiles];
+#pragma unroll
+  for (int t = 0; t < kRowTiles; ++t) bfr[0][t] = *reinterpret_cast<const bf16x8_t *>(act + act_off(t * 16 + r, g));
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    if (ks + 1 < KS) {
+#pragma unroll
+      for (int t = 0; t < kRowTiles; ++t)
+        bfr[(ks + 1) & 1][t] = *reinterpret_cast<const bf16x8_t *>(act + act_off(t * 16 + r, (ks + 1) * 4 + g));
+    }
+    __builtin_amdgcn_sched_barrier(0);  // the next k-step's LDS reads go first
+#pragma unroll
+    for (int i = 0; i < kTilesPerWave; ++i) {
+#pragma unroll
+      for (int t = 0; t < kRowTiles; ++t)
+        acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring[i], bfr[ks & 1][t], acc[i][t], 0, 0, 0);
+      const int sofs = ks + 1 < KS ? lbytes + frag_bytes(ks + 1, ft0 + i) : next_lbytes + frag_bytes(0, ft0 + i);
+      ring[i] = load_frag(w, lane * 16, sofs);
+    }
+    // keep the next k-step's loads in this one: left free, the scheduler sinks each load
+    // to just before its use and the MFMAs wait on L2 latency
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Hidden layer: h = SiLU(z + b) and, for the tangent rows, t * SiLU'(z), stored over the
+// layer's inputs in LDS (every wave has finished reading them).  Lane (r, g) holds
+// features 16 ft + 4 g + e of row 16 t + r for its tiles ft.
+template <int NT>
+__device__ __forceinline__ void hidden_epilogue(const f32x4_t (&acc)[kTilesPerWave][kRowTiles],
+                                                const float *bias_l, __bf16 *act, int ft0, int lane_in) {
+  constexpr int G = kRowTiles / (1 + NT);
+  const int lane = opaque_lane(lane_in);
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < kTilesPerWave; ++i) {
+    const int f0 = (ft0 + i) * 16 + 4 * g;
+    const f32x4_t bv = *reinterpret_cast<const f32x4_t *>(bias_l + f0);
+    const int off = act_off(r, f0 >> 3) + (f0 & 7);  // row tile t: + 16 t rows
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const int b0 = q * (1 + NT);
+      bf16x4_t h, tg[NT > 0 ? NT : 1];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float z = acc[i][b0][e] + bv[e];
+        const float s = sigmoid_f(z);
+        h[e] = (__bf16)(z * s);
+        if constexpr (NT > 0) {
+          const float d = s * (1.0f + z * (1.0f - s));
+#pragma unroll
+          for (int a = 0; a < NT; ++a) tg[a][e] = (__bf16)(acc[i][b0 + 1 + a][e] * d);
+        }
+      }
+      *reinterpret_cast<bf16x4_t *>(act + off + b0 * 16 * kWidth) = h;
+      if constexpr (NT > 0) {
+#pragma unroll
+        for (int a = 0; a < NT; ++a) *reinterpret_cast<bf16x4_t *>(act + off + (b0 + 1 + a) * 16 * kWidth) = tg[a];
+      }
+    }
+  }
+}
+
+// Last hidden layer: this lane's partial sums of the output layer, in FP32.
+template <int NT, int O>
+__device__ __forceinline__ void output_partials(const f32x4_t (&acc)[kTilesPerWave][kRowTiles],
+                                                const float *bias_l, const float *wout, int ft0, int lane_in,
+                                                float (&yp)[kRowTiles][O]) {
+  constexpr int G = kRowTiles / (1 + NT);
+  const int g = opaque_lane(lane_in) >> 4;
+#pragma unroll
+  for (int t = 0; t < kRowTiles; ++t)
+#pragma unroll
+    for (int o = 0; o < O; ++o) yp[t][o] = 0.0f;
+#pragma unroll
+  for (int i = 0; i < kTilesPerWave; ++i) {
+    const int f0 = (ft0 + i) * 16 + 4 * g;
+    const f32x4_t bv = *reinterpret_cast<const f32x4_t *>(bias_l + f0);
+    f32x4_t wv[O];
+#pragma unroll
+    for (int o = 0; o < O; ++o) wv[o] = *reinterpret_cast<const f32x4_t *>(wout + o * kWidth + f0);
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const int b0 = q * (1 + NT);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const float z = acc[i][b0][e] + bv[e];
+        const float s = sigmoid_f(z);
+        const float hz = z * s;
+        const float d = s * (1.0f + z * (1.0f - s));
+#pragma unroll
+        for (int o = 0; o < O; ++o) {
+          yp[b0][o] = fmaf(hz, wv[o][e], yp[b0][o]);
+#pragma unroll
+          for (int a = 1; a <= NT; ++a) yp[b0 + a][o] = fmaf(acc[i][b0 + a][e] * d, wv[o][e], yp[b0 + a][o]);
+        }
+      }
+    }
+  }
+}
+
+// Fourier features of the tile's seeds (and their input derivatives) into LDS rows:
+// features [sin | cos] of the 64 frequencies, one task per (seed, block of 8 frequencies).
+template <int NT>
+__device__ __forceinline__ void fourier_rows(const WideArgs &a, int64_t tile, const float *freq_s,
+                                             const float *dfreq_s, __bf16 *act, int tid) {
+  constexpr int S = kRows / (1 + NT);
+  for (int task = tid; task < S * 8; task += kThreads) {
+    const int j = task >> 3, fb = task & 7;
+    const int64_t seed = tile * S + j;
+    float xin[kMaxIn];
+#pragma unroll
+    for (int c = 0; c < kMaxIn; ++c) xin[c] = (seed < a.M && c < a.d_in) ? a.x[seed * a.d_in + c] : 0.0f;
+    bf16x8_t sn, cs, tsn[NT > 0 ? NT : 1], tcs[NT > 0 ? NT : 1];
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+      const int f = fb * 8 + jj;
+      const f32x4_t fr = *reinterpret_cast<const f32x4_t *>(freq_s + f * kMaxIn);
+      float ph = 0.0f;  // revolutions
+#pragma unroll
+      for (int c = 0; c < kMaxIn; ++c) ph += fr[c] * xin[c];
+      ph -= rintf(ph);
+      const float s = __builtin_amdgcn_sinf(ph), c = __builtin_amdgcn_cosf(ph);
+      sn[jj] = (__bf16)s;
+      cs[jj] = (__bf16)c;
+      if constexpr (NT > 0) {
+        const f32x4_t dq = *reinterpret_cast<const f32x4_t *>(dfreq_s + f * kMaxIn);
+#pragma unroll
+        for (int a2 = 0; a2 < NT; ++a2) {
+          tsn[a2][jj] = (__bf16)(c * dq[a2]);
+          tcs[a2][jj] = (__bf16)(-s * dq[a2]);
+        }
+      }
+    }
+    const int row0 = (j >> 4) * (1 + NT) * 16 + (j & 15);
+    *reinterpret_cast<bf16x8_t *>(act + act_off(row0, fb)) = sn;
+    *reinterpret_cast<bf16x8_t *>(act + act_off(row0, kNFreq / 8 + fb)) = cs;
+    if constexpr (NT > 0) {
+#pragma unroll
+      for (int a2 = 0; a2 < NT; ++a2) {
+        const int row = row0 + (1 + a2) * 16;
+        *reinterpret_cast<bf16x8_t *>(act + act_off(row, fb)) = tsn[a2];
+        *reinterpret_cast<bf16x8_t *>(act + act_off(row, kNFreq / 8 + fb)) = tcs[a2];
+      }
+    }
+  }
+}
+
+// Persistent: work-group b handles tiles b, b + gridDim.x, ...
+template <int NT, int O>
+__global__ void __launch_bounds__(kThreads, 2) wide_mlp_kernel(WideArgs a) {
+  constexpr int S = kRows / (1 + NT);  // seeds per tile
+  __shared__ __attribute__((aligned(16))) __bf16 act[kRows * kWidth];
+  __shared__ __attribute__((aligned(16))) float freq_s[kNFreq * kMaxIn];   // B * scale, zero-padded
+  __shared__ __attribute__((aligned(16))) float dfreq_s[kNFreq * kMaxIn];  // 2 pi B * scale
+  __shared__ float red[kWaves][kRows][4];                                  // output partials per wave
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ft0 = wave * kTilesPerWave;
+  const int64_t ntiles = ((int64_t)a.M + S - 1) / S;
+  {
+    const int q = tid >> 2, c = tid & 3;  // kThreads == kNFreq * kMaxIn
+    float fv = 0.0f, sc = 0.0f;
+    if (c < a.d_in) {
+      fv = a.Bf[q * a.d_in + c];
+      sc = a.scale[c];
+    }
+    freq_s[tid] = fv * sc;
+    dfreq_s[tid] = 6.283185307179586f * fv * sc;
+  }
+  const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void *)a.W, 0, a.w_bytes, 0x00020000);
+  bf16x8_t ring[kTilesPerWave];
+#pragma unroll
+  for (int i = 0; i < kTilesPerWave; ++i) ring[i] = load_frag(wr, lane * 16, frag_bytes(0, ft0 + i));
+  __syncthreads();
+
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    fourier_rows<NT>(a, tile, freq_s, dfreq_s, act, tid);
+    __syncthreads();
+    f32x4_t acc[kTilesPerWave][kRowTiles];
+    for (int L = 0; L < a.depth; ++L) {
+      const int lb = layer_bytes(L), nlb = layer_bytes(L + 1 < a.depth ? L + 1 : 0);
+      if (L == 0) layer_mma<kFeat>(wr, lb, nlb, act, ring, acc, ft0, lane);
+      else layer_mma<kWidth>(wr, lb, nlb, act, ring, acc, ft0, lane);
+      const float *bl = a.bias + (int64_t)L * kWidth;
+      if (L + 1 < a.depth) {
+        __syncthreads();  // every wave has read this layer's inputs
+        hidden_epilogue<NT>(acc, bl, act, ft0, lane);
+        __syncthreads();
+      } else {
+        float yp[kRowTiles][O];
+        output_partials<NT, O>(acc, bl, a.Wout, ft0, lane, yp);
+        // the 4 lanes of a row (lane % 16 equal) hold disjoint feature subsets
+#pragma unroll
+        for (int t = 0; t < kRowTiles; ++t)
+#pragma unroll
+          for (int o = 0; o < O; ++o) {
+            float v = yp[t][o];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            yp[t][o] = v;
+          }
+        if (lane < 16) {
+#pragma unroll
+          for (int t = 0; t < kRowTiles; ++t)
+#pragma unroll
+            for (int o = 0; o < O; ++o) red[wave][t * 16 + lane][o] = yp[t][o];
+        }
+        // also orders this tile's reads of `act` before the next tile's Fourier rows
+        __syncthreads();
+        const int n = tid >> 2, o = tid & 3;  // kThreads == kRows * 4
+        const int t = n >> 4, q = t / (1 + NT), a2 = t % (1 + NT);
+        const int64_t seed = tile * S + q * 16 + (n & 15);
+        if (o < O && seed < a.M) {
+          float v = red[0][n][o];
+#pragma unroll
+          for (int w = 1; w < kWaves; ++w) v += red[w][n][o];
+          a.Y[((int64_t)a2 * a.M + seed) * O + o] = (a2 == 0 ? v + a.bout[o] : v) * a.out_scale;
+        }
+      }
+    }
+  }
+}
+
+static_assert(kThreads == kNFreq * kMaxIn && kThreads == kRows * 4, "one thread per staged entry");
+
+__global__ void pack_wide_kernel(const float *W, int rows, int K, __bf16 *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)rows * K) return;
+  const int j = (int)(i & 7), lane = (int)((i >> 3) & 63);
+  const int64_t frag = i >> 9;
+  const int tiles = rows / 16;
+  const int ks = (int)(frag / tiles), ft = (int)(frag % tiles);
+  const int m = ft * 16 + (lane & 15), k = ks * 32 + 8 * (lane >> 4) + j;
+  out[i] = (__bf16)W[(int64_t)m * K + k];
+}
+
+template <int NT, int O>
+void launch_wide(const WideArgs &a, hipStream_t st) {
+  // two work-groups per CU (LDS and registers admit two); each loops over tiles
+  constexpr int64_t S = kRows / (1 + NT);
+  const int64_t tiles = ((int64_t)a.M + S - 1) / S;
+  const unsigned grid = (unsigned)std::min<int64_t>(tiles, 2 * (int64_t)device_cu_count());
+  wide_mlp_kernel<NT, O><<<grid, kThreads, 0, st>>>(a);
+}
+
+}  // namespace
+}  // namespace pyftle
+
+using namespace pyftle;
+
+extern "C" {
+
+int pyftle_mlp_pack_wide(const float *W, int32_t rows, int32_t K, void *out, void *stream) {
+  if (!W || !out) return fail("mlp_pack_wide: NULL pointer");
+  if (rows < 16 || rows % 16 || K < 32 || K % 32)
+    return fail("mlp_pack_wide: rows must be a positive multiple of 16 and K of 32");
+  const int64_t n = (int64_t)rows * K;
+  pack_wide_kernel<<<(unsigned)((n + 255) / 256), 256, 0, as_stream(stream)>>>(W, rows, K, (__bf16 *)out);
+  return check_launch("mlp_pack_wide");
+}
+
+int pyftle_mlp_wide_forward(const float *x, int32_t d_in, int32_t M, const float *scale, const float *Bf,
+                            int32_t n_freq, const void *W_packed, const float *bias, int32_t depth,
+                            int32_t width, const float *Wout, const float *bout, int32_t O, float out_scale,
+                            int32_t n_tangents, float *Y, void *stream) {
+  if (!x || !scale || !Bf || !W_packed || !bias || !Wout || !bout || !Y)
+    return fail("mlp_wide_forward: NULL pointer");
+  if (n_freq != kNFreq || width != kWidth)
+    return fail("mlp_wide_forward: needs 64 Fourier frequencies and hidden width 512");
+  if (depth < 2 || depth > kMaxDepth) return fail("mlp_wide_forward: depth must be in [2, 8]");
+  if (d_in < 1 || d_in > kMaxIn || M < 0 || n_tangents < 0 || n_tangents > d_in)
+    return fail("mlp_wide_forward: bad input sizes");
+  if ((uintptr_t)W_packed % 16 || (uintptr_t)bias % 16 || (uintptr_t)Wout % 16)
+    return fail("mlp_wide_forward: packed weights, bias and Wout must be 16-byte aligned");
+  if (M == 0) return PYFTLE_OK;
+  const int w_bytes = 2 * (kWidth * kFeat + (depth - 1) * kWidth * kWidth);
+  WideArgs a{x, scale, Bf, (const __bf16 *)W_packed, w_bytes, bias, Wout, bout, Y, out_scale, d_in, M, depth};
+  hipStream_t st = as_stream(stream);
+  if (O == 2 && n_tangents == 0) launch_wide<0, 2>(a, st);
+  else if (O == 3 && n_tangents == 0) launch_wide<0, 3>(a, st);
+  else if (O == 3 && n_tangents == 3) launch_wide<3, 3>(a, st);
+  else return fail("mlp_wide_forward: supported (outputs, tangents): (2, 0), (3, 0), (3, 3)");
+  return check_launch("mlp_wide_forward");
+}
+
+}  // extern "C"

@@ -45,7 +45,12 @@ _SIGS = {
                                   _vp, _vp],
     "pyftle_mlp_fused_forward": [_vp, _i32, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _f32,
                                  _i32, _vp, _vp],
+    "pyftle_mlp_pack_wide": [_vp, _i32, _i32, _vp, _vp],
+    "pyftle_mlp_wide_forward": [_vp, _i32, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _f32,
+                                _i32, _vp, _vp],
 }
+# (outputs, tangents) covered by the one-kernel networks, per hidden width
+_FUSED_SHAPES = {256: ((2, 0), (2, 2), (3, 0)), 512: ((2, 0), (3, 0), (3, 3))}
 _registered = False
 _WORKSPACE: dict = {}
 
@@ -170,9 +175,11 @@ class FlowMapMLP:
         }
 
     def fused_supported(self, n_tangents: int) -> bool:
+        """Whether a one-kernel network covers this shape: width 256 keeps the activations in
+        registers (``mlp_fused.hip``), width 512 in LDS (``mlp_wide.hip``)."""
         s = self.spec
-        return (s.n_freq == 64 and s.width == 256 and 2 <= s.depth <= 8 and s.d_in <= 4
-                and (s.d_out, n_tangents) in ((2, 0), (2, 2), (3, 0)))
+        return (s.n_freq == 64 and 2 <= s.depth <= 8 and s.d_in <= 4
+                and (s.d_out, n_tangents) in _FUSED_SHAPES.get(s.width, ()))
 
     def pack_fused(self):
         """Weights in the fused kernel's order (bf16) and the hidden biases (depth, width)."""
@@ -181,11 +188,15 @@ class FlowMapMLP:
         Wp = torch.empty(n, dtype=torch.bfloat16, device=self.params.device)
         pos = 0
         for l, (o, (rows, k), _) in enumerate(self.offsets):
-            N.call("pyftle_mlp_pack_fused", _off(self.params, o), rows, k, int(l > 0), _off(Wp, pos),
-                   stream_handle())
+            if s.width == 512:
+                N.call("pyftle_mlp_pack_wide", _off(self.params, o), rows, k, _off(Wp, pos), stream_handle())
+            else:
+                N.call("pyftle_mlp_pack_fused", _off(self.params, o), rows, k, int(l > 0), _off(Wp, pos),
+                       stream_handle())
             pos += rows * k
         bias = torch.stack([self.b(l) for l in range(s.depth)]).contiguous()
-        self._fused = (Wp, bias, self.Wout().contiguous(), self.bout().contiguous())
+        # own allocations: the wide kernel reads bias and Wout with 16-byte vector loads
+        self._fused = (Wp, bias, self.Wout().clone(), self.bout().clone())
         self._fused_step = self.step
         return self._fused
 
@@ -209,9 +220,9 @@ class FlowMapMLP:
             raise ValueError(f"out must be a contiguous float32 tensor of shape {shape} on {x.device}")
         else:
             Y = out
-        N.call("pyftle_mlp_fused_forward", _p(x), s.d_in, M, _p(self.scale), _p(self.B), s.n_freq, _p(Wp),
-               _p(bias), s.depth, s.width, _p(wout), _p(bout), s.d_out, float(s.out_scale), int(n_tangents),
-               _p(Y), stream_handle())
+        entry = "pyftle_mlp_wide_forward" if s.width == 512 else "pyftle_mlp_fused_forward"
+        N.call(entry, _p(x), s.d_in, M, _p(self.scale), _p(self.B), s.n_freq, _p(Wp), _p(bias), s.depth,
+               s.width, _p(wout), _p(bout), s.d_out, float(s.out_scale), int(n_tangents), _p(Y), stream_handle())
         return Y
 
     def state_dict(self):

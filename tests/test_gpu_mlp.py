@@ -203,6 +203,49 @@ def test_fused_three_outputs_and_repack_after_update():
     assert _rel(m.forward_fused(xt)[0].cpu().numpy(), MO.forward(p2, x)) < 3e-2
 
 
+def _wide_model(seed=11, depth=3, d_out=3):
+    from pyftle_amd.surrogate import FlowMapMLP, MLPSpec
+
+    spec = MLPSpec(d_in=4, d_out=d_out, n_freq=64, width=512, depth=depth, sigma=0.5,
+                   scale=(0.5, 0.5, 0.5, 0.1), out_scale=2.0)
+    return FlowMapMLP(spec, seed=seed)
+
+
+@pytest.mark.parametrize("M", [1, 15, 17, 300, 5000])
+def test_wide_forward_and_jvp_match_oracle(M):
+    """Width-512 one-kernel network (activations in LDS) vs FP64: primal and 3 tangents;
+    M covers partial tiles of 16 and 64 seeds."""
+    m = _wide_model()
+    p = m.to_numpy()
+    rng = np.random.default_rng(M)
+    x = rng.uniform([0, 0, 0, -1], [6.3, 6.3, 6.3, 11], (M, 4))
+    xt = torch.as_tensor(x, dtype=torch.float32, device="cuda")
+    Y = m.forward_fused(xt, n_tangents=3).cpu().numpy()
+    assert Y.shape == (4, M, 3)
+    assert _rel(Y[0], MO.forward(p, x)) < 3e-2
+    for a in range(3):
+        assert _rel(Y[1 + a], MO.jvp(p, x, a)) < 3e-2
+    Y0 = m.forward_fused(xt).cpu().numpy()
+    assert np.array_equal(Y0[0], Y[0])  # the primal rows do not depend on the tangents
+
+
+def test_wide_matches_layered_path_and_two_outputs():
+    m = _wide_model(seed=12, depth=6)
+    g = torch.Generator(device="cuda").manual_seed(4)
+    x = torch.rand(3000, 4, device="cuda", generator=g) * torch.tensor([6.3, 6.3, 6.3, 10.0], device="cuda")
+    y, tans = m.forward(x, n_tangents=3)
+    Y = m.forward_fused(x, n_tangents=3)
+    assert _rel(Y[0].cpu().numpy(), y.cpu().numpy()) < 2e-2
+    for a in range(3):
+        assert _rel(Y[1 + a].cpu().numpy(), tans[a].cpu().numpy()) < 2e-2
+    m2 = _wide_model(seed=13, depth=2, d_out=2)
+    p2 = m2.to_numpy()
+    x2 = np.random.default_rng(5).uniform([0, 0, 0, 0], [2, 1, 1, 10], (700, 4))
+    Y2 = m2.forward_fused(torch.as_tensor(x2, dtype=torch.float32, device="cuda"))
+    assert Y2.shape == (1, 700, 2)
+    assert _rel(Y2[0].cpu().numpy(), MO.forward(p2, x2)) < 3e-2
+
+
 def test_fused_rejects_unsupported_shapes():
     m = _small_model()  # width 128
     with pytest.raises(ValueError, match="fused inference"):
