@@ -467,7 +467,7 @@ def run_surrogate3d(torch, wl, args):
             "per_t0": per_t0,
             "mfma": {"achieved": rows4 * fpq / (jm4 * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": rows4 * fpq / (jm4 * 1e-3) / 1e12 / 2500.0, "flops_per_query": fpq,
-                     "kernel": "gemm_bf16_kernel (layer by layer, width 512)"},
+                     "kernel": "fused_mlp_kernel<512,4,3,3> and <512,4,0,3>"},
             "train_s": t_train, "data_s": t_data},
         "rk4": {"ms_all_seeds_one_window": rk4_ms, "seeds_per_s": n_seeds / (rk4_ms * 1e-3)},
         "speedup_vs_gpu_rk4": rk4_ms / sur_ms,
@@ -568,9 +568,10 @@ def run_surrogate_infer(torch, wl, args):
         "warmup": 1, "ms_per_step": main["ms"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16", "data": "synthetic", "config": wl["config"],
         "roofline": {"bound": "mfma", "achieved": ach, "peak": 2500.0, "unit": "TFLOP/s", "frac": ach / 2500.0,
-                     "traffic": None, "kernel": "fused_mlp_kernel<1,2,2> (2M seeds, primal + 2 tangent rows)",
+                     "traffic": None,
+                     "kernel": "fused_mlp_kernel<256,6,2,2> (2M seeds, primal + 2 tangent rows)",
                      "flops_per_launch": kflops, "avg_launch_ms": kms},
-        "fused_primal_only": {"kernel": "fused_mlp_kernel<3,0,2>", "avg_launch_ms": kms0,
+        "fused_primal_only": {"kernel": "fused_mlp_kernel<256,8,0,2>", "avg_launch_ms": kms0,
                               "tflops": seeds.shape[0] * fpq / (kms0 * 1e-3) / 1e12},
         "ensemble": {str(k): v for k, v in out.items()},
         "cpu_baseline": cpu,

@@ -351,30 +351,19 @@ int pyftle_mlp_adam_scheduled(float *w, float *m, float *v, const float *g, int6
 int pyftle_mlp_cast_bf16(const float *x, int64_t n, void *y, void *stream);
 
 /* Fused inference (Fourier features, every hidden layer and the output layer in one
- * kernel, activations kept in registers): 64 frequencies, hidden width 256, depth 2..8,
- * (outputs, tangents) in {(2, 0), (2, 2), (3, 0)}.  W_packed: layer 0 (256 x 128) then
- * layers 1.. (256 x 256), bf16, as written by pyftle_mlp_pack_fused (permute = 0 for
- * layer 0, 1 for the others); bias (depth x 256); Y (1 + n_tangents, M, O) with the
- * output scale applied. */
-int pyftle_mlp_pack_fused(const float *W, int32_t rows, int32_t K, int32_t permute, void *out,
-                          void *stream);
+ * kernel; each work-group keeps its rows' activations in LDS and streams the weights from
+ * L2 into registers): 64 frequencies, hidden width 256 or 512, depth 2..8, (width,
+ * outputs, tangents) in {(256, 2, 0), (256, 2, 2), (256, 3, 0), (512, 2, 0), (512, 3, 0),
+ * (512, 3, 3)}.  W_packed: every layer (width x K, layer 0 with K = 128 first) as written
+ * by pyftle_mlp_pack_fused (rows a multiple of 16, K of 32); W_packed, bias (depth x width)
+ * and Wout (O x width) 16-byte aligned; Y (1 + n_tangents, M, O) with the output scale
+ * applied. */
+int pyftle_mlp_pack_fused(const float *W, int32_t rows, int32_t K, void *out, void *stream);
 int pyftle_mlp_fused_forward(const float *x, int32_t d_in, int32_t M, const float *scale,
                              const float *Bf, int32_t n_freq, const void *W_packed,
                              const float *bias, int32_t depth, int32_t width, const float *Wout,
                              const float *bout, int32_t O, float out_scale, int32_t n_tangents,
                              float *Y, void *stream);
-
-/* Fused inference for hidden width 512 (activations kept in LDS, weights streamed from L2
- * into registers): 64 frequencies, depth 2..8, (outputs, tangents) in {(2, 0), (3, 0),
- * (3, 3)}.  W_packed: every layer (512 x K) as written by pyftle_mlp_pack_wide (rows a
- * multiple of 16, K of 32), layer 0 (K = 128) first; W_packed, bias (depth x 512) and Wout
- * (O x 512) 16-byte aligned; Y (1 + n_tangents, M, O) with the output scale applied. */
-int pyftle_mlp_pack_wide(const float *W, int32_t rows, int32_t K, void *out, void *stream);
-int pyftle_mlp_wide_forward(const float *x, int32_t d_in, int32_t M, const float *scale,
-                            const float *Bf, int32_t n_freq, const void *W_packed,
-                            const float *bias, int32_t depth, int32_t width, const float *Wout,
-                            const float *bout, int32_t O, float out_scale, int32_t n_tangents,
-                            float *Y, void *stream);
 
 #ifdef __cplusplus
 }

@@ -40,16 +40,13 @@ _SIGS = {
     "pyftle_mlp_mse": [_vp, _vp, _i64, _f32, _vp, _vp, _vp],
     "pyftle_mlp_adam": [_vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _i32, _f32, _vp, _vp],
     "pyftle_mlp_cast_bf16": [_vp, _i64, _vp, _vp],
-    "pyftle_mlp_pack_fused": [_vp, _i32, _i32, _i32, _vp, _vp],
+    "pyftle_mlp_pack_fused": [_vp, _i32, _i32, _vp, _vp],
     "pyftle_mlp_adam_scheduled": [_vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _f32, _f32, _i32, _i32, _vp,
                                   _vp, _vp],
     "pyftle_mlp_fused_forward": [_vp, _i32, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _f32,
                                  _i32, _vp, _vp],
-    "pyftle_mlp_pack_wide": [_vp, _i32, _i32, _vp, _vp],
-    "pyftle_mlp_wide_forward": [_vp, _i32, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _f32,
-                                _i32, _vp, _vp],
 }
-# (outputs, tangents) covered by the one-kernel networks, per hidden width
+# (outputs, tangents) covered by the one-kernel network, per hidden width
 _FUSED_SHAPES = {256: ((2, 0), (2, 2), (3, 0)), 512: ((2, 0), (3, 0), (3, 3))}
 _registered = False
 _WORKSPACE: dict = {}
@@ -175,8 +172,7 @@ class FlowMapMLP:
         }
 
     def fused_supported(self, n_tangents: int) -> bool:
-        """Whether a one-kernel network covers this shape: width 256 keeps the activations in
-        registers (``mlp_fused.hip``), width 512 in LDS (``mlp_wide.hip``)."""
+        """Whether the one-kernel network (``mlp_fused.hip``) covers this shape."""
         s = self.spec
         return (s.n_freq == 64 and 2 <= s.depth <= 8 and s.d_in <= 4
                 and (s.d_out, n_tangents) in _FUSED_SHAPES.get(s.width, ()))
@@ -187,15 +183,11 @@ class FlowMapMLP:
         n = sum(shp[0] * shp[1] for _, shp, _ in self.offsets)
         Wp = torch.empty(n, dtype=torch.bfloat16, device=self.params.device)
         pos = 0
-        for l, (o, (rows, k), _) in enumerate(self.offsets):
-            if s.width == 512:
-                N.call("pyftle_mlp_pack_wide", _off(self.params, o), rows, k, _off(Wp, pos), stream_handle())
-            else:
-                N.call("pyftle_mlp_pack_fused", _off(self.params, o), rows, k, int(l > 0), _off(Wp, pos),
-                       stream_handle())
+        for o, (rows, k), _ in self.offsets:
+            N.call("pyftle_mlp_pack_fused", _off(self.params, o), rows, k, _off(Wp, pos), stream_handle())
             pos += rows * k
         bias = torch.stack([self.b(l) for l in range(s.depth)]).contiguous()
-        # own allocations: the wide kernel reads bias and Wout with 16-byte vector loads
+        # own allocations: the kernel reads bias and Wout with 16-byte vector loads
         self._fused = (Wp, bias, self.Wout().clone(), self.bout().clone())
         self._fused_step = self.step
         return self._fused
@@ -220,8 +212,7 @@ class FlowMapMLP:
             raise ValueError(f"out must be a contiguous float32 tensor of shape {shape} on {x.device}")
         else:
             Y = out
-        entry = "pyftle_mlp_wide_forward" if s.width == 512 else "pyftle_mlp_fused_forward"
-        N.call(entry, _p(x), s.d_in, M, _p(self.scale), _p(self.B), s.n_freq, _p(Wp), _p(bias), s.depth,
+        N.call("pyftle_mlp_fused_forward", _p(x), s.d_in, M, _p(self.scale), _p(self.B), s.n_freq, _p(Wp), _p(bias), s.depth,
                s.width, _p(wout), _p(bout), s.d_out, float(s.out_scale), int(n_tangents), _p(Y), stream_handle())
         return Y
 
