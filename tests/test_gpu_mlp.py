@@ -246,6 +246,32 @@ def test_wide_matches_layered_path_and_two_outputs():
     assert _rel(Y2[0].cpu().numpy(), MO.forward(p2, x2)) < 3e-2
 
 
+@pytest.mark.parametrize("width,d_in,nt", [(512, 4, 3), (512, 4, 0), (256, 3, 2), (256, 3, 0)])
+def test_fused_persistent_tiles_match_layered_path(width, d_in, nt):
+    """Enough seeds that every work-group loops over several tiles (the weight prefetch
+    carries over from one tile's last layer to the next tile's first)."""
+    from pyftle_amd.surrogate import FlowMapMLP, MLPSpec
+
+    d_out = 3 if width == 512 else 2
+    spec = MLPSpec(d_in=d_in, d_out=d_out, n_freq=64, width=width, depth=3, sigma=0.5,
+                   scale=(0.5,) * (d_in - 1) + (0.1,), out_scale=1.5)
+    m = FlowMapMLP(spec, seed=21)
+    M = 70001
+    g = torch.Generator(device="cuda").manual_seed(6)
+    x = torch.rand(M, d_in, device="cuda", generator=g) * 6.0
+    Y = m.forward_fused(x, n_tangents=nt)
+    if nt:
+        y, tans = m.forward(x, n_tangents=nt)
+        refs = [y] + list(tans)
+    else:
+        refs = [m.forward(x)]
+    for k, ref in enumerate(refs):
+        assert _rel(Y[k].cpu().numpy(), ref.cpu().numpy()) < 2e-2
+    # row by row: the last tiles agree as well as the first
+    tail = slice(M - 100, M)
+    assert _rel(Y[0][tail].cpu().numpy(), refs[0][tail].cpu().numpy()) < 2e-2
+
+
 def test_fused_rejects_unsupported_shapes():
     m = _small_model()  # width 128
     with pytest.raises(ValueError, match="fused inference"):
