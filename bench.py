@@ -579,8 +579,86 @@ def run_surrogate_infer(torch, wl, args):
     print(json.dumps(result), flush=True)
 
 
+def build_surrogate3d_infer(torch, ns=256, members=4):
+    """Inference half of surrogate3d: one window's 256^3 seeds through four random-init
+    6x512 members (no field, no training), for profiling the width-512 fused kernel."""
+    return dict(name="surrogate3d_infer", ns=ns, members=members, T=5.0,
+                config={"workload": "surrogate3d_infer", "seeds": [ns, ns, ns], "t0": 2.5,
+                        "mlp": "64 Fourier freqs (sigma 0.5) on (x,y,z,t0), 6x512 SiLU -> 3",
+                        "members": members, "weights": "random init (no training)",
+                        "dtype": "bf16 MFMA, fp32 accumulate"})
+
+
+def run_surrogate3d_infer(torch, wl, args):
+    """Ensemble FTLE (no fallback) of one 256^3-seed window, and the width-512 fused kernel
+    alone on 2^19 seeds, timed with HIP events on the current stream."""
+    from pyftle_amd import surrogate as S
+    from pyftle_amd import surrogate_ftle as SF
+
+    spec = S.MLPSpec(d_in=4, d_out=3, n_freq=64, width=512, depth=6, sigma=0.5, scale=(0.5, 0.5, 0.5, 0.1),
+                     out_scale=2.0)
+    ens = S.FlowMapEnsemble([S.FlowMapMLP(spec, seed=1000 + k) for k in range(wl["members"])])
+    ns, L = wl["ns"], 2.0 * math.pi
+    dev = torch.device("cuda")
+    sax = (torch.arange(ns, device=dev, dtype=torch.float64) + 0.5) * (L / ns)
+    cz, cy, cx = torch.meshgrid(sax, sax, sax, indexing="ij")
+    seeds = torch.stack([cx.reshape(-1), cy.reshape(-1), cz.reshape(-1)], 1).contiguous()
+    del cx, cy, cz
+    sf = SF.SurrogateFTLE(ens, None, wl["T"], 0.01, tol=float("inf"), chunk=1 << 19)
+    fpq = S.mlp_flops_per_query(spec)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    steps = max(1, args.steps)
+    n = seeds.shape[0]
+    out = {}
+    for jm in (wl["members"], 1):
+        sf.jacobian_members = jm
+        ms = timed(lambda: sf.predict(seeds, 2.5), steps)
+        rows = n * (jm * 4 + (wl["members"] - jm))
+        out[jm] = {"ms": ms, "seeds_per_s": n / (ms * 1e-3), "tflops": rows * fpq / (ms * 1e-3) / 1e12}
+        log(f"ensemble FTLE, Jacobian from {jm} member(s): {ms:.1f} ms, {out[jm]['tflops']:.0f} TFLOP/s")
+    m0 = ens.members[0]
+    M = 1 << 19
+    x = torch.empty(M, 4, dtype=torch.float32, device=dev)
+    x[:, :3] = seeds[:M].float()
+    x[:, 3] = 2.5
+    Y = torch.empty(4, M, 3, dtype=torch.float32, device=dev)
+    kms = timed(lambda: m0.forward_fused(x, n_tangents=3, out=Y), 10 * steps)
+    Y0 = torch.empty(1, M, 3, dtype=torch.float32, device=dev)
+    kms0 = timed(lambda: m0.forward_fused(x, n_tangents=0, out=Y0), 10 * steps)
+    kflops = M * 4 * fpq
+    ach = kflops / (kms * 1e-3) / 1e12
+    main = out[wl["members"]]
+    result = {
+        "metric": METRIC, "value": main["seeds_per_s"], "unit": "seeds/s", "n_gpus": 1, "steps": steps,
+        "warmup": 1, "ms_per_step": main["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16", "data": "synthetic", "config": wl["config"],
+        "roofline": {"bound": "mfma", "achieved": ach, "peak": 2500.0, "unit": "TFLOP/s", "frac": ach / 2500.0,
+                     "traffic": None,
+                     "kernel": "fused_mlp_kernel<512,4,3,3> (2^19 seeds, primal + 3 tangent rows)",
+                     "flops_per_launch": kflops, "avg_launch_ms": kms},
+        "fused_primal_only": {"kernel": "fused_mlp_kernel<512,4,0,3>", "avg_launch_ms": kms0,
+                              "tflops": M * fpq / (kms0 * 1e-3) / 1e12},
+        "ensemble": {str(k): v for k, v in out.items()},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(result), flush=True)
+
+
 WORKLOADS = {"abc128": build_abc128, "gyre512": build_gyre, "batch64": build_batch64,
              "surrogate2d": build_surrogate2d, "surrogate_infer": build_surrogate_infer,
+             "surrogate3d_infer": build_surrogate3d_infer,
              "surrogate3d": build_surrogate3d}
 
 
@@ -928,6 +1006,8 @@ def main():
         return run_surrogate(torch, wl, args)
     if wl["name"] == "surrogate_infer":
         return run_surrogate_infer(torch, wl, args)
+    if wl["name"] == "surrogate3d_infer":
+        return run_surrogate3d_infer(torch, wl, args)
     if wl["name"] == "surrogate3d":
         return run_surrogate3d(torch, wl, args)
     cfg = best_config(shape_class(wl["dim"], wl["n_seeds"], wl["time_mode"]))
