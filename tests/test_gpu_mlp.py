@@ -165,7 +165,7 @@ def _fused_model(seed=4, depth=3, d_out=2):
 
 @pytest.mark.parametrize("M", [1, 37, 300, 4096])
 def test_fused_forward_and_jvp_match_oracle(M):
-    """One-kernel network (weights permuted for the register-resident layout) vs FP64."""
+    """One-kernel network at width 256 (activations in LDS, 2 tangents) vs FP64."""
     m = _fused_model()
     p = m.to_numpy()
     rng = np.random.default_rng(M)
@@ -213,7 +213,7 @@ def _wide_model(seed=11, depth=3, d_out=3):
 
 @pytest.mark.parametrize("M", [1, 15, 17, 300, 5000])
 def test_wide_forward_and_jvp_match_oracle(M):
-    """Width-512 one-kernel network (activations in LDS) vs FP64: primal and 3 tangents;
+    """The one-kernel network at width 512 vs FP64: primal and 3 tangents;
     M covers partial tiles of 16 and 64 seeds."""
     m = _wide_model()
     p = m.to_numpy()
