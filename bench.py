@@ -41,6 +41,24 @@ def log(msg):
 # workloads
 # ---------------------------------------------------------------------------
 
+# record layout of the FP32 fields: "node" (one record per grid node, the layout of the
+# reference's arrays), "cell" (one record per cell holding its 2^dim corners) or "auto"
+# (cell in 3D: 6 contiguous 16-byte loads per trilinear sample instead of 8 scattered
+# ones, abc128 226 -> 187 ms; node in 2D, where a cell record is as many loads and the
+# 4x larger slot costs cache hits: gyre512 1.22 -> 1.42 ms, batch64 536 -> 601 ms)
+RECORDS = "auto"
+
+
+def _layout(N, pair, dim):
+    if RECORDS == "cell" or (RECORDS == "auto" and dim == 3):
+        return N.LAYOUT_CELL_PAIR if pair else N.LAYOUT_CELL
+    return N.LAYOUT_PAIR if pair else N.LAYOUT_SINGLE
+
+
+def _records_name(dim):
+    return "cell" if RECORDS == "cell" or (RECORDS == "auto" and dim == 3) else "node"
+
+
 def build_abc128(torch, n=128, ns=128, delta=1e-3, h=0.01, steps=1000):
     import numpy as np
 
@@ -50,7 +68,7 @@ def build_abc128(torch, n=128, ns=128, delta=1e-3, h=0.01, steps=1000):
     L = 2.0 * math.pi
     sp = L / n
     geom = GridGeometry.uniform((n, n, n), (0.0, 0.0, 0.0), (sp, sp, sp), periodic=(True, True, True))
-    field = DeviceField(geom, dtype="float32", layout=N.LAYOUT_SINGLE, ring_slots=1, n_snapshots=1)
+    field = DeviceField(geom, dtype="float32", layout=_layout(N, False, 3), ring_slots=1, n_snapshots=1)
     ax = np.arange(n) * sp
     Z, Y, X = np.meshgrid(ax, ax, ax, indexing="ij")  # flat ix + n*(iy + n*iz)
     A, B, C = math.sqrt(3.0), math.sqrt(2.0), 1.0
@@ -96,7 +114,7 @@ def build_gyre(torch, nx=512, ny=256, n_snap=101, nsx=400, nsy=200, delta=1e-3, 
     from pyftle_amd.field import DeviceField, GridGeometry
 
     geom = GridGeometry.uniform((nx, ny), (0.0, 0.0), (2.0 / (nx - 1), 1.0 / (ny - 1)))
-    field = DeviceField(geom, dtype="float32", layout=N.LAYOUT_PAIR, ring_slots=n_snap,
+    field = DeviceField(geom, dtype="float32", layout=_layout(N, True, 2), ring_slots=n_snap,
                        n_snapshots=n_snap, t0=0.0, dt_snap=0.1)
     x = np.linspace(0.0, 2.0, nx)
     y = np.linspace(0.0, 1.0, ny)
@@ -151,8 +169,8 @@ def _gyre_field_on_device(torch, nx, ny, n_snap, eps, omega, dt_snap=0.1, A=0.1,
     from pyftle_amd.field import DeviceField, GridGeometry
 
     geom = GridGeometry.uniform((nx, ny), (0.0, 0.0), (2.0 / (nx - 1), 1.0 / (ny - 1)))
-    field = DeviceField(geom, dtype=dtype, layout=N.LAYOUT_PAIR, ring_slots=n_snap,
-                        n_snapshots=n_snap, t0=0.0, dt_snap=dt_snap)
+    field = DeviceField(geom, dtype=dtype, layout=_layout(N, True, 2) if dtype == "float32" else N.LAYOUT_PAIR,
+                        ring_slots=n_snap, n_snapshots=n_snap, t0=0.0, dt_snap=dt_snap)
     dev = torch.device("cuda")
     x = torch.linspace(0.0, 2.0, nx, dtype=torch.float64, device=dev)
     y = torch.linspace(0.0, 1.0, ny, dtype=torch.float64, device=dev)
@@ -669,10 +687,13 @@ WORKLOADS = {"abc128": build_abc128, "gyre512": build_gyre, "batch64": build_bat
 def accuracy_vs_fp64(torch, W, wl, ftle32, cfg):
     """Same window in FP64 on the GPU (the kernel that matches the oracle bit
     for bit on small cases), over the full field."""
+    from pyftle_amd import _native as N
     from pyftle_amd.field import DeviceField
 
     f32 = wl["field"]
-    f64 = DeviceField(f32.geom, dtype="float64", layout=f32.layout, ring_slots=f32.ring_slots,
+    # the node layout of the same time levels (the cell layouts hold FP32 records only)
+    lay = N.LAYOUT_PAIR if f32.layout in (N.LAYOUT_PAIR, N.LAYOUT_CELL_PAIR) else N.LAYOUT_SINGLE
+    f64 = DeviceField(f32.geom, dtype="float64", layout=lay, ring_slots=f32.ring_slots,
                       n_snapshots=f32.n_snapshots, t0=f32.t0, dt_snap=f32.dt_snap)
     if wl["name"] == "abc128":
         f64.pack(0, wl["host"]["vel"])
@@ -963,6 +984,10 @@ def main():
     ap.add_argument("--seed-order", default="auto", choices=["auto", "morton", "none"],
                     help="abc128 / gyre512: process seed groups in Morton order of their cells (results are "
                          "identical; computed inside every timed step); auto = from 2^18 seed groups")
+    ap.add_argument("--records", default="auto", choices=["auto", "node", "cell"],
+                    help="record layout of the FP32 fields of abc128, gyre512, batch64 and the 2D surrogate "
+                         "workloads: one record per grid node, one per cell holding its corners, or auto "
+                         "(cell in 3D, node in 2D)")
     ap.add_argument("--groups-per-block", type=int, default=0)
     ap.add_argument("--lane-order", type=int, default=-1)
     ap.add_argument("--xcd-remap", type=int, default=-1)
@@ -999,7 +1024,11 @@ def main():
         kw["epochs"] = args.surrogate_epochs
     if args.workload == "surrogate2d" and args.surrogate_depth > 0:
         kw["depth"] = args.surrogate_depth
+    global RECORDS
+    RECORDS = args.records
     wl = WORKLOADS[args.workload](torch, **kw)
+    if "config" in wl and args.workload not in ("surrogate3d", "surrogate3d_infer"):
+        wl["config"]["records"] = _records_name(wl.get("dim", 2))
     if wl["name"] == "batch64":
         return run_batch(torch, wl, args)
     if wl["name"] == "surrogate2d":
@@ -1057,8 +1086,8 @@ def main():
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic.get("hbm_bytes_per_launch") if traffic else None,
-            "kernel": "advect_kernel_cell<3,RK4,STEADY,SINGLE,FTLE>" if wl["dim"] == 3 else
-                      "advect_kernel_cell<2,RK4,LINEAR,PAIR,FTLE>",
+            "kernel": (f"advect_kernel_cell<3,RK4,STEADY,FTLE> ({_records_name(3)} records)" if wl["dim"] == 3
+                       else f"advect_kernel_cell<2,RK4,LINEAR,PAIR,FTLE> ({_records_name(2)} records)"),
             "algorithmic_bytes_per_launch": alg_bytes,
             "bytes_per_seed_step": wl["bytes_per_seed_step"],
             "avg_launch_ms": avg_ms}

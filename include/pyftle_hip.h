@@ -48,6 +48,14 @@ extern "C" {
 /* ---- snapshot record layouts in the device ring ---- */
 #define PYFTLE_LAYOUT_SINGLE 0 /* record = one time level: 2D (u,v); 3D (u,v,w,0) */
 #define PYFTLE_LAYOUT_PAIR 1   /* record = levels k,k+1: 2D (u0,v0,u1,v1); 3D (u0,v0,w0,0,u1,v1,w1,0) */
+/* Per-cell records (FP32 GridInterp-mode fields, read by the FP32 advection and batch
+ * kernels only): the record of cell (i, j[, k]) holds the 2^dim corner nodes of the cell,
+ * corner q = (i + (q&1), j + (q>>1 & 1)[, k + (q>>2 & 1)]), dim floats each, corner after
+ * corner; one level (CELL) or levels k, k+1 one after the other (CELL_PAIR).  n cells per
+ * axis (the last one holds valid corners on periodic axes only).  One interpolation reads
+ * one contiguous record (96 bytes in 3D) instead of 2^dim node records. */
+#define PYFTLE_LAYOUT_CELL 2
+#define PYFTLE_LAYOUT_CELL_PAIR 3
 
 /* ---- integrators (integrate.py:350-387 names) ---- */
 #define PYFTLE_EULER 0
@@ -132,7 +140,7 @@ typedef struct pyftle_field {
   double spacing[3];    /* (max - min)/(n - 1) per axis (grid_interp.py:34-35) */
   int32_t periodic[3];  /* 1: index wraps modulo n (extension); 0: zero outside (GridInterp.cpp:58,126) */
   int32_t dtype;        /* PYFTLE_F32 or PYFTLE_F64 record components; also the compute precision */
-  int32_t layout;       /* PYFTLE_LAYOUT_SINGLE or PYFTLE_LAYOUT_PAIR */
+  int32_t layout;       /* PYFTLE_LAYOUT_* */
   int32_t ring_slots;   /* number of resident slots (>= 1) */
   int32_t n_snapshots;  /* snapshots in the time series (time clamping in LINEAR mode) */
   int64_t slot_stride;  /* bytes between slots (>= record bytes * nodes) */

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_lib", "libpyftle_hip.so")
 
 F32, F64 = 0, 1
-LAYOUT_SINGLE, LAYOUT_PAIR = 0, 1
+LAYOUT_SINGLE, LAYOUT_PAIR, LAYOUT_CELL, LAYOUT_CELL_PAIR = 0, 1, 2, 3
 EULER, AB2, RK4 = 0, 1, 2
 TIME_STEADY, TIME_FROZEN, TIME_LINEAR = 0, 1, 2
 ORDER_SEED_MAJOR, ORDER_NEIGHBOR_MAJOR = 0, 1

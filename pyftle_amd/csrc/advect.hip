@@ -16,7 +16,7 @@ static int check_opts(const pyftle_field *field, const pyftle_advect_opts *o) {
   if (o->n_steps < 0) return fail("advect: n_steps must be >= 0");
   if (!std::isfinite(o->h)) return fail("advect: h must be finite");
   if (o->time_mode == PYFTLE_TIME_LINEAR) {
-    if (field->layout != PYFTLE_LAYOUT_PAIR) return fail("advect: LINEAR time needs the PAIR layout");
+    if (!layout_pair(field->layout)) return fail("advect: LINEAR time needs the PAIR or CELL_PAIR layout");
     if (!(field->dt_snap != 0.0) || !std::isfinite(field->dt_snap) || !std::isfinite(field->t0))
       return fail("advect: LINEAR time needs finite t0 and non-zero dt_snap");
     if (!std::isfinite(o->t_start)) return fail("advect: t_start must be finite");
@@ -34,6 +34,8 @@ static int check_opts(const pyftle_field *field, const pyftle_advect_opts *o) {
     return fail("advect: unknown compute precision");
   if (field->dtype == PYFTLE_F64 && o->compute != PYFTLE_COMPUTE_NATIVE)
     return fail("advect: an FP64 field always computes in FP64 (compute must be NATIVE)");
+  if (layout_cell(field->layout) && o->compute != PYFTLE_COMPUTE_NATIVE)
+    return fail("advect: the CELL layouts are read by the FP32 kernel only (compute must be NATIVE)");
   if (field->interp != PYFTLE_INTERP_GINTERP && o->time_mode == PYFTLE_TIME_LINEAR)
     return fail("advect: RegularGridInterpolator modes run STEADY or FROZEN time");
   return PYFTLE_OK;
@@ -130,9 +132,9 @@ int pyftle_advect(const pyftle_field *field, const pyftle_advect_opts *opts, voi
   a.order = opts->seed_order;
   const int64_t blocks = (n_particles + a.G - 1) / a.G;
   if (blocks > INT32_MAX) return fail("advect: too many particles");
-  LaunchSpec sp{opts->method, opts->time_mode, field->layout == PYFTLE_LAYOUT_PAIR,
+  LaunchSpec sp{opts->method, opts->time_mode, layout_pair(field->layout),
                 field->dtype == PYFTLE_F64, prec_of(opts), 0, (unsigned)blocks, 256u,
-                as_stream(stream)};
+                as_stream(stream), layout_cell(field->layout)};
   dispatch(field, a, sp);
   return check_launch("advect");
 }
@@ -175,9 +177,9 @@ int pyftle_advect_ftle(const pyftle_field *field, const pyftle_advect_opts *opts
   if (blocks > INT32_MAX) return fail("advect_ftle: too many seeds");
   int threads = G * nb;
   threads = ((threads + 63) / 64) * 64;
-  LaunchSpec sp{opts->method, opts->time_mode, field->layout == PYFTLE_LAYOUT_PAIR,
+  LaunchSpec sp{opts->method, opts->time_mode, layout_pair(field->layout),
                 field->dtype == PYFTLE_F64, prec_of(opts), 1, (unsigned)blocks, (unsigned)threads,
-                as_stream(stream)};
+                as_stream(stream), layout_cell(field->layout)};
   dispatch(field, a, sp);
   return check_launch("advect_ftle");
 }

@@ -390,7 +390,7 @@ __device__ __forceinline__ int cell_axis(const FieldDev &f, int a, int cell, flo
   return i;
 }
 
-template <int DIM, bool PAIR>
+template <int DIM, bool PAIR, bool CELL>
 __device__ __forceinline__ void sample_cell(const FieldDev &f, __amdgpu_buffer_rsrc_t rs, float w,
                                             const int *cell, const float *t, float *vel) {
   constexpr int C = RecInfo<DIM, PAIR>::comps;
@@ -409,11 +409,27 @@ __device__ __forceinline__ void sample_cell(const FieldDev &f, __amdgpu_buffer_r
   if constexpr (DIM == 3) base += (int)__umul24((unsigned)i0[2], (unsigned)f.pxy);
   constexpr int NQ = (DIM == 2) ? 4 : 8;
   float c[NQ][C];
+  if constexpr (CELL) {
+    // one record per cell: level l, corner q, component comp at float (l NQ + q) DIM + comp
+    constexpr int LV = PAIR ? 2 : 1;
+    constexpr int RF = NQ * DIM * LV;  // 8, 16, 24 or 48 floats
+    float rec[RF];
+    const unsigned roff = (unsigned)base * (unsigned)(RF * 4);  // out-of-range offsets read 0
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    const int node = base + ((q & 1) ? 1 : 0) + ((q & 2) ? f.px : 0) + ((q & 4) ? f.pxy : 0);
+    for (int h = 0; h < RF / 4; ++h) BufVec<4>::load(rs, (int)(roff + 16u * h), &rec[h * 4]);
 #pragma unroll
-    for (int h = 0; h < chunks; ++h) BufVec<NF>::load(rs, node * RB + h * 16, &c[q][h * 4]);
+    for (int l = 0; l < LV; ++l)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int comp = 0; comp < DIM; ++comp) c[q][l * L + comp] = rec[(l * NQ + q) * DIM + comp];
+  } else {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int node = base + ((q & 1) ? 1 : 0) + ((q & 2) ? f.px : 0) + ((q & 4) ? f.pxy : 0);
+#pragma unroll
+      for (int h = 0; h < chunks; ++h) BufVec<NF>::load(rs, node * RB + h * 16, &c[q][h * 4]);
+    }
   }
   const float xd = d[0], yd = d[1], zd = d[DIM - 1];
 #pragma unroll
@@ -455,7 +471,7 @@ __device__ __forceinline__ int wrap_cell(int &cell, int n, float inv_n) {
 }
 
 // One work-group's share of a window: logical block lb of the seeds of `a`.
-template <int DIM, int METHOD, int TMODE, bool PAIR, bool FTLE>
+template <int DIM, int METHOD, int TMODE, bool PAIR, bool FTLE, bool CELL>
 __device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t lb) {
   const int t = threadIdx.x;
   const int nb = a.nb, G = a.G;
@@ -542,18 +558,18 @@ __device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t l
     double inc[DIM];
 #pragma unroll
     for (int c = 0; c < DIM; ++c) tp[c] = frac[c];
-    sample_cell<DIM, PAIR>(f, r0, (float)w0, cell, tp, k1);
+    sample_cell<DIM, PAIR, CELL>(f, r0, (float)w0, cell, tp, k1);
     if (METHOD == PYFTLE_RK4) {
       float k2[DIM], k3[DIM], k4[DIM];
 #pragma unroll
       for (int c = 0; c < DIM; ++c) tp[c] = fmaf(sh[c], k1[c], frac[c]);
-      sample_cell<DIM, PAIR>(f, r1, (float)w1, cell, tp, k2);
+      sample_cell<DIM, PAIR, CELL>(f, r1, (float)w1, cell, tp, k2);
 #pragma unroll
       for (int c = 0; c < DIM; ++c) tp[c] = fmaf(sh[c], k2[c], frac[c]);
-      sample_cell<DIM, PAIR>(f, r1, (float)w1, cell, tp, k3);
+      sample_cell<DIM, PAIR, CELL>(f, r1, (float)w1, cell, tp, k3);
 #pragma unroll
       for (int c = 0; c < DIM; ++c) tp[c] = fmaf(sf[c], k3[c], frac[c]);
-      sample_cell<DIM, PAIR>(f, r2, (float)w2, cell, tp, k4);
+      sample_cell<DIM, PAIR, CELL>(f, r2, (float)w2, cell, tp, k4);
 #pragma unroll
       for (int c = 0; c < DIM; ++c)
         inc[c] = s6d[c] * ((double)k1[c] + 2.0 * ((double)k2[c] + (double)k3[c]) + (double)k4[c]);
@@ -619,9 +635,9 @@ __device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t l
   if constexpr (FTLE) ftle_epilogue<DIM>(a, lb, t, sl, j, fin, x0);
 }
 
-template <int DIM, int METHOD, int TMODE, bool PAIR, bool FTLE>
+template <int DIM, int METHOD, int TMODE, bool PAIR, bool FTLE, bool CELL>
 __global__ void __launch_bounds__(512) advect_kernel_cell(AdvectArgs a) {
-  advect_block_cell<DIM, METHOD, TMODE, PAIR, FTLE>(a, logical_block(a.remap));
+  advect_block_cell<DIM, METHOD, TMODE, PAIR, FTLE, CELL>(a, logical_block(a.remap));
 }
 
 // ---------------------------------------------------------------------------
@@ -632,15 +648,22 @@ struct LaunchSpec {
   int method, tmode, pair, store64, prec, ftle;
   unsigned blocks, threads;
   hipStream_t stream;
+  int cell;  // per-cell records (FP32, PREC_NATIVE only)
 };
 
 template <int DIM, int METHOD, int TMODE, bool PAIR, bool STORE64, int PREC>
 void launch_leaf(const AdvectArgs &a, const LaunchSpec &sp) {
   if constexpr (!STORE64 && PREC == PREC_NATIVE) {
-    if (sp.ftle)
-      advect_kernel_cell<DIM, METHOD, TMODE, PAIR, true><<<sp.blocks, sp.threads, 0, sp.stream>>>(a);
-    else
-      advect_kernel_cell<DIM, METHOD, TMODE, PAIR, false><<<sp.blocks, sp.threads, 0, sp.stream>>>(a);
+    if (sp.cell) {
+      if (sp.ftle)
+        advect_kernel_cell<DIM, METHOD, TMODE, PAIR, true, true><<<sp.blocks, sp.threads, 0, sp.stream>>>(a);
+      else
+        advect_kernel_cell<DIM, METHOD, TMODE, PAIR, false, true><<<sp.blocks, sp.threads, 0, sp.stream>>>(a);
+    } else if (sp.ftle) {
+      advect_kernel_cell<DIM, METHOD, TMODE, PAIR, true, false><<<sp.blocks, sp.threads, 0, sp.stream>>>(a);
+    } else {
+      advect_kernel_cell<DIM, METHOD, TMODE, PAIR, false, false><<<sp.blocks, sp.threads, 0, sp.stream>>>(a);
+    }
   } else {
     if (sp.ftle)
       advect_kernel<DIM, METHOD, TMODE, PAIR, STORE64, PREC, true><<<sp.blocks, sp.threads, 0, sp.stream>>>(a);

@@ -11,7 +11,7 @@
 #include "advect_impl.hpp"
 
 struct pyftle_batch {
-  int dim = 0, method = 0, tmode = 0, pair = 0;
+  int dim = 0, method = 0, tmode = 0, pair = 0, cell = 0;
   int threads = 0, grid = 0;
   int64_t n_items = 0;
   pyftle::AdvectArgs *d_table = nullptr;
@@ -27,7 +27,7 @@ namespace pyftle {
 
 __global__ void reset_counter_kernel(unsigned int *c) { *c = 0u; }
 
-template <int DIM, int METHOD, int TMODE, bool PAIR>
+template <int DIM, int METHOD, int TMODE, bool PAIR, bool CELL>
 __global__ void __launch_bounds__(512) batch_kernel(const AdvectArgs *__restrict__ table,
                                                     const int2 *__restrict__ items, int n_items,
                                                     unsigned int *counter) {
@@ -40,37 +40,38 @@ __global__ void __launch_bounds__(512) batch_kernel(const AdvectArgs *__restrict
     if (it >= n_items) break;
     const int f = __builtin_amdgcn_readfirstlane(items[it].x);
     const int b = __builtin_amdgcn_readfirstlane(items[it].y);
-    advect_block_cell<DIM, METHOD, TMODE, PAIR, true>(table[f], b);
+    advect_block_cell<DIM, METHOD, TMODE, PAIR, true, CELL>(table[f], b);
   }
 }
 
 using BatchFn = void (*)(const AdvectArgs *, const int2 *, int, unsigned int *);
 
 template <int DIM, int METHOD, int TMODE, bool PAIR>
-static const void *kernel_ptr() {
-  return reinterpret_cast<const void *>(&batch_kernel<DIM, METHOD, TMODE, PAIR>);
+static const void *kernel_ptr(int cell) {
+  return cell ? reinterpret_cast<const void *>(&batch_kernel<DIM, METHOD, TMODE, PAIR, true>)
+              : reinterpret_cast<const void *>(&batch_kernel<DIM, METHOD, TMODE, PAIR, false>);
 }
 
 template <int DIM, int METHOD>
-static const void *pick_time(int tmode, int pair) {
-  if (tmode == PYFTLE_TIME_LINEAR) return kernel_ptr<DIM, METHOD, PYFTLE_TIME_LINEAR, true>();
+static const void *pick_time(int tmode, int pair, int cell) {
+  if (tmode == PYFTLE_TIME_LINEAR) return kernel_ptr<DIM, METHOD, PYFTLE_TIME_LINEAR, true>(cell);
   if (tmode == PYFTLE_TIME_FROZEN)
-    return pair ? kernel_ptr<DIM, METHOD, PYFTLE_TIME_FROZEN, true>()
-                : kernel_ptr<DIM, METHOD, PYFTLE_TIME_FROZEN, false>();
-  return pair ? kernel_ptr<DIM, METHOD, PYFTLE_TIME_STEADY, true>()
-              : kernel_ptr<DIM, METHOD, PYFTLE_TIME_STEADY, false>();
+    return pair ? kernel_ptr<DIM, METHOD, PYFTLE_TIME_FROZEN, true>(cell)
+                : kernel_ptr<DIM, METHOD, PYFTLE_TIME_FROZEN, false>(cell);
+  return pair ? kernel_ptr<DIM, METHOD, PYFTLE_TIME_STEADY, true>(cell)
+              : kernel_ptr<DIM, METHOD, PYFTLE_TIME_STEADY, false>(cell);
 }
 
 template <int DIM>
-static const void *pick_method(int method, int tmode, int pair) {
-  if (method == PYFTLE_RK4) return pick_time<DIM, PYFTLE_RK4>(tmode, pair);
-  if (method == PYFTLE_EULER) return pick_time<DIM, PYFTLE_EULER>(tmode, pair);
-  return pick_time<DIM, PYFTLE_AB2>(tmode, pair);
+static const void *pick_method(int method, int tmode, int pair, int cell) {
+  if (method == PYFTLE_RK4) return pick_time<DIM, PYFTLE_RK4>(tmode, pair, cell);
+  if (method == PYFTLE_EULER) return pick_time<DIM, PYFTLE_EULER>(tmode, pair, cell);
+  return pick_time<DIM, PYFTLE_AB2>(tmode, pair, cell);
 }
 
 static const void *batch_kernel_for(const pyftle_batch *b) {
-  return b->dim == 2 ? pick_method<2>(b->method, b->tmode, b->pair)
-                     : pick_method<3>(b->method, b->tmode, b->pair);
+  return b->dim == 2 ? pick_method<2>(b->method, b->tmode, b->pair, b->cell)
+                     : pick_method<3>(b->method, b->tmode, b->pair, b->cell);
 }
 
 static int enqueue(pyftle_batch *b, hipStream_t st) {
@@ -135,8 +136,8 @@ int pyftle_batch_create(const pyftle_batch_field *fields, int32_t n_fields,
     if (bf.field.interp != PYFTLE_INTERP_GINTERP)
       return fail("batch_create: batches run GridInterp-mode fields (interp 0)");
     if (opts->time_mode == PYFTLE_TIME_LINEAR &&
-        (layout != PYFTLE_LAYOUT_PAIR || !(bf.field.dt_snap != 0.0)))
-      return fail("batch_create: LINEAR time needs PAIR fields with non-zero dt_snap");
+        (!layout_pair(layout) || !(bf.field.dt_snap != 0.0)))
+      return fail("batch_create: LINEAR time needs PAIR or CELL_PAIR fields with non-zero dt_snap");
     if (opts->time_mode == PYFTLE_TIME_FROZEN &&
         (bf.first_snapshot < 0 || (int64_t)bf.first_snapshot + bf.n_steps > bf.field.n_snapshots))
       return fail("batch_create: FROZEN steps run past the last snapshot of field " + std::to_string(i));
@@ -182,7 +183,8 @@ int pyftle_batch_create(const pyftle_batch_field *fields, int32_t n_fields,
   b->dim = dim;
   b->method = opts->method;
   b->tmode = opts->time_mode;
-  b->pair = layout == PYFTLE_LAYOUT_PAIR;
+  b->pair = layout_pair(layout);
+  b->cell = layout_cell(layout);
   b->threads = ((G * nb + 63) / 64) * 64;
   b->n_items = n_items;
   b->use_graph = opts->use_graph != 0;

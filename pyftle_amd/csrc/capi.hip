@@ -72,8 +72,10 @@ int validate_field(const pyftle_field *f) {
     if (!isfinite(f->origin[a])) return fail("field.origin must be finite");
   }
   if (f->dtype != PYFTLE_F32 && f->dtype != PYFTLE_F64) return fail("field.dtype must be F32 or F64");
-  if (f->layout != PYFTLE_LAYOUT_SINGLE && f->layout != PYFTLE_LAYOUT_PAIR)
-    return fail("field.layout must be SINGLE or PAIR");
+  if (f->layout < PYFTLE_LAYOUT_SINGLE || f->layout > PYFTLE_LAYOUT_CELL_PAIR)
+    return fail("field.layout must be SINGLE, PAIR, CELL or CELL_PAIR");
+  if (layout_cell(f->layout) && (f->dtype != PYFTLE_F32 || f->interp != PYFTLE_INTERP_GINTERP))
+    return fail("the CELL layouts hold FP32 GridInterp-mode fields");
   if (f->interp < PYFTLE_INTERP_GINTERP || f->interp > PYFTLE_INTERP_SCATTER_CUBIC)
     return fail("field.interp must be one of PYFTLE_INTERP_*");
   if (f->interp != PYFTLE_INTERP_GINTERP && !scatter) {

@@ -83,10 +83,15 @@ inline bool is_scatter(int interp) {
 int make_field_dev(const pyftle_field *f, FieldDev *out);
 int validate_field(const pyftle_field *f);
 
+// two time levels per record (PAIR, CELL_PAIR); per-cell records (CELL, CELL_PAIR)
+inline bool layout_pair(int layout) { return (layout & 1) != 0; }
+inline bool layout_cell(int layout) { return (layout & 2) != 0; }
+
 // records per axis of one slot: n, plus the ghost node of a periodic axis, or
-// n + 2 spline coefficients (-1..n) in the cubic mode
+// n + 2 spline coefficients (-1..n) in the cubic mode, or n cells
 inline int axis_records(const pyftle_field *f, int a) {
   if (f->interp == PYFTLE_INTERP_RGI_CUBIC) return f->n[a] + 2;
+  if (layout_cell(f->layout)) return f->n[a];
   return f->n[a] + (f->periodic[a] ? 1 : 0);
 }
 
@@ -102,7 +107,8 @@ inline int64_t padded_nodes(const pyftle_field *f) {
 // components per point (value and gradient of u and v)
 inline int record_bytes(int dim, int layout, int dtype, int interp = PYFTLE_INTERP_GINTERP) {
   if (interp == PYFTLE_INTERP_SCATTER_CUBIC) return 8 * (dtype == PYFTLE_F64 ? 8 : 4);
-  int comps = (dim == 2 ? 2 : 4) * (layout == PYFTLE_LAYOUT_PAIR ? 2 : 1);
+  const int levels = layout_pair(layout) ? 2 : 1;
+  const int comps = layout_cell(layout) ? (1 << dim) * dim * levels : (dim == 2 ? 2 : 4) * levels;
   return comps * (dtype == PYFTLE_F64 ? 8 : 4);
 }
 

@@ -99,6 +99,43 @@ __global__ void __launch_bounds__(256) pack_kernel(char *__restrict__ slot_a, in
   }
 }
 
+// Snapshot ingest into per-cell records (PYFTLE_LAYOUT_CELL, CELL_PAIR): one lane per cell
+// writes the dim components of the cell's 2^dim corner nodes, corner after corner, into one
+// time level.  Upper corners wrap on periodic axes; on the others the last cell (never read
+// as valid) repeats the last node.
+template <int DIM, typename Src>
+__global__ void __launch_bounds__(256) pack_cell_kernel(char *__restrict__ slot_a, int level_a,
+                                                        char *__restrict__ slot_b, int level_b,
+                                                        const Src *__restrict__ src, int64_t s0, int64_t s1,
+                                                        int64_t s2, int64_t s3, int nx, int ny, int nz,
+                                                        int px, int py, int pz, int rec_floats) {
+  constexpr int NQ = 1 << DIM;
+  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= (int64_t)nx * ny * nz) return;
+  const int ix = (int)(cell % nx);
+  const int iy = (int)((cell / nx) % ny);
+  const int iz = (int)(cell / ((int64_t)nx * ny));
+  auto up = [](int i, int n, int periodic) { return i + 1 < n ? i + 1 : (periodic ? 0 : n - 1); };
+  float vals[NQ * DIM];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int jx = (q & 1) ? up(ix, nx, px) : ix;
+    const int jy = (q & 2) ? up(iy, ny, py) : iy;
+    const int jz = (DIM == 3 && (q & 4)) ? up(iz, nz, pz) : iz;
+    const int64_t base = (int64_t)jx * s1 + (int64_t)jy * s2 + (int64_t)jz * s3;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) vals[q * DIM + c] = (float)src[base + (int64_t)c * s0];
+  }
+  float *ra = reinterpret_cast<float *>(slot_a) + cell * rec_floats + level_a * NQ * DIM;
+#pragma unroll
+  for (int e = 0; e < NQ * DIM; ++e) ra[e] = vals[e];
+  if (slot_b) {
+    float *rb = reinterpret_cast<float *>(slot_b) + cell * rec_floats + level_b * NQ * DIM;
+#pragma unroll
+    for (int e = 0; e < NQ * DIM; ++e) rb[e] = vals[e];
+  }
+}
+
 // Physical-space sampling, one lane per point.
 template <int DIM, bool PAIR, typename Real, typename P>
 __global__ void __launch_bounds__(256) sample_kernel(FieldDev f, const char *slot, double w,
@@ -221,7 +258,7 @@ int pyftle_pack_snapshot(const pyftle_field *field, int32_t snapshot, const void
   const int dim = field->dim;
   const int nz = dim == 3 ? field->n[2] : 1;
   const int per_level = dim == 2 ? 2 : 4;
-  const int pair = field->layout == PYFTLE_LAYOUT_PAIR;
+  const int pair = layout_pair(field->layout);
   const int comps = per_level * (pair ? 2 : 1);
   char *base = reinterpret_cast<char *>(field->data);
   auto slot = [&](int k) {
@@ -255,8 +292,21 @@ int pyftle_pack_snapshot(const pyftle_field *field, int32_t snapshot, const void
   const int gz = nz + (dim == 3 && field->periodic[2] ? 1 : 0);
   const int64_t nodes = (int64_t)gx * gy * gz;
   hipStream_t st = as_stream(stream);
+  const int cells = field->n[0] * field->n[1] * nz;  // < 2^30 (validate_field)
+  const int rec_floats = (1 << dim) * dim * (pair ? 2 : 1);
   auto launch = [&](char *sa, int lva, char *sb, int lvb) {
-    if (field->dtype == PYFTLE_F32) {
+    if (layout_cell(field->layout)) {
+      const int px = field->periodic[0], py = field->periodic[1], pz = dim == 3 ? field->periodic[2] : 0;
+#define PYFTLE_PACK_CELL(D, S)                                                                        \
+  pack_cell_kernel<D, S><<<blocks_for(cells, 256), 256, 0, st>>>(                                    \
+      sa, lva, sb, lvb, (const S *)src, src_strides[0], src_strides[1], src_strides[2], src_strides[3], \
+      field->n[0], field->n[1], nz, px, py, pz, rec_floats)
+      if (dim == 2 && src_dtype == PYFTLE_F64) PYFTLE_PACK_CELL(2, double);
+      else if (dim == 2) PYFTLE_PACK_CELL(2, float);
+      else if (src_dtype == PYFTLE_F64) PYFTLE_PACK_CELL(3, double);
+      else PYFTLE_PACK_CELL(3, float);
+#undef PYFTLE_PACK_CELL
+    } else if (field->dtype == PYFTLE_F32) {
       if (src_dtype == PYFTLE_F64)
         pack_kernel<float, double><<<blocks_for(nodes, 256), 256, 0, st>>>(
             sa, lva, sb, lvb, (const double *)src, src_strides[0], src_strides[1], src_strides[2],
@@ -292,7 +342,9 @@ int pyftle_field_sample(const pyftle_field *field, int32_t time_mode, int32_t sn
   if (n == 0) return PYFTLE_OK;
   int k = snapshot;
   double w = 0.0;
-  const bool pair = field->layout == PYFTLE_LAYOUT_PAIR;
+  if (layout_cell(field->layout))
+    return fail("field_sample: the CELL layouts are read by the FP32 advection kernels only");
+  const bool pair = layout_pair(field->layout);
   if (time_mode == PYFTLE_TIME_LINEAR) {
     if (!pair) return fail("field_sample: LINEAR time needs the PAIR layout");
     const double s = (t - field->t0) / field->dt_snap;

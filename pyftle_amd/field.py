@@ -287,14 +287,19 @@ class DeviceField:
         self.t0 = float(t0)
         self.dt_snap = float(dt_snap)
         dim = geom.dim
-        comps = (2 if dim == 2 else 4) * (2 if layout == N.LAYOUT_PAIR else 1)
+        levels = 2 if layout in (N.LAYOUT_PAIR, N.LAYOUT_CELL_PAIR) else 1
+        cell = layout in (N.LAYOUT_CELL, N.LAYOUT_CELL_PAIR)
+        # per-cell records hold the 2^dim corners of a cell (dim components each)
+        comps = ((1 << dim) * dim if cell else (2 if dim == 2 else 4)) * levels
         if self.interp == N.INTERP_SCATTER_CUBIC:
             comps = 8  # value and gradient of u and v, padded
         esize = 4 if self.dtype == torch.float32 else 8
         # a periodic axis stores one ghost node (a copy of node 0) at index n; the cubic
-        # mode stores spline coefficients -1..n per axis
+        # mode stores spline coefficients -1..n per axis; the cell layouts n cells per axis
         if self.interp == N.INTERP_RGI_CUBIC:
             nodes = int(np.prod([geom.n[a] + 2 for a in range(dim)]))
+        elif cell:
+            nodes = int(np.prod([geom.n[a] for a in range(dim)]))
         else:
             nodes = int(np.prod([geom.n[a] + (1 if geom.periodic[a] else 0) for a in range(dim)]))
         self.record_bytes = comps * esize

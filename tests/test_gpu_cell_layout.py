@@ -1,0 +1,145 @@
+"""GPU: the per-cell record layouts (CELL, CELL_PAIR) against the node layouts (SINGLE, PAIR).
+
+A cell record holds the same FP32 values as the 2^dim node records of the cell and the
+kernel does the same arithmetic on them, so final positions and FTLE are bit-identical:
+periodic and bounded axes, particles that leave the domain (zero velocity outside), every
+integrator, steady / frozen / time-linear snapshots, single launches and batches.  The
+layouts are read by the FP32 advection kernels only; other uses are refused.
+"""
+
+import numpy as np
+import pytest
+
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _lib():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from pyftle_amd import _native
+
+    _native.load()
+
+
+def _field(vels, pts, shape, layout, periodic=None, dt_snap=0.1, precision="float32"):
+    from pyftle_amd.field import DeviceField, GridGeometry
+
+    kw = {} if periodic is None else {"periodic": periodic}
+    f = DeviceField(GridGeometry.from_reference(shape, pts, **kw), dtype=precision, layout=layout,
+                    ring_slots=len(vels), n_snapshots=len(vels), t0=0.0, dt_snap=dt_snap)
+    for k, v in enumerate(vels):
+        f.pack(k, v)
+    return f
+
+
+def _run(f, x0, **kw):
+    from pyftle_amd import window as W
+
+    pos = torch.as_tensor(x0, device="cuda")
+    out = torch.empty_like(pos)
+    ftle = W.advect_ftle(f, pos, positions_out=out, **kw)
+    return ftle, out
+
+
+def _seeds_2d_with_outside():
+    """Seed groups over the domain [0, 2] x [0, 1] and a band around it: some start
+    outside, some on the upper boundary, some leave during the window."""
+    x0 = H.seeds_2d(23, 11, lo=(-0.1, -0.05), hi=(2.1, 1.0))
+    return x0
+
+
+@pytest.mark.parametrize("method", ["euler", "ab2", "rk4"])
+def test_cell_layout_2d_bounded_bit_identical(method):
+    from pyftle_amd import _native as N
+
+    vels, pts = H.double_gyre_window(nx=48, ny=24, n_snap=7)
+    x0 = _seeds_2d_with_outside()
+    node = _field(vels, pts, (48, 24), N.LAYOUT_SINGLE)
+    cell = _field(vels, pts, (48, 24), N.LAYOUT_CELL)
+    for kw in (dict(time_mode="frozen", n_steps=6, h=0.1, map_period=0.5),
+               dict(time_mode="steady", n_steps=40, h=-0.05, map_period=2.0)):
+        f0, p0 = _run(node, x0, method=method, **kw)
+        f1, p1 = _run(cell, x0, method=method, **kw)
+        assert torch.equal(p1, p0)
+        assert torch.equal(f1, f0)
+        assert not torch.equal(p0, torch.as_tensor(x0, device="cuda"))  # the particles moved
+
+
+def test_cell_pair_layout_2d_time_linear_bit_identical():
+    from pyftle_amd import _native as N
+
+    vels, pts = H.double_gyre_window(nx=64, ny=32, n_snap=11)
+    x0 = _seeds_2d_with_outside()
+    node = _field(vels, pts, (64, 32), N.LAYOUT_PAIR)
+    cell = _field(vels, pts, (64, 32), N.LAYOUT_CELL_PAIR)
+    for t_start, h in ((0.0, 0.05), (0.93, -0.04)):
+        kw = dict(method="rk4", time_mode="linear", n_steps=20, h=h, t_start=t_start, map_period=20 * abs(h))
+        f0, p0 = _run(node, x0, **kw)
+        f1, p1 = _run(cell, x0, **kw)
+        assert torch.equal(p1, p0)
+        assert torch.equal(f1, f0)
+
+
+@pytest.mark.parametrize("unsteady", [False, True])
+def test_cell_layouts_3d_periodic_bit_identical(unsteady):
+    from pyftle_amd import _native as N
+
+    n = 20
+    vels, pts = H.abc_window(n=n, n_snap=5 if unsteady else 1, snap_dt=0.25, unsteady=unsteady)
+    per = (True, True, True)
+    lay0, lay1 = (N.LAYOUT_PAIR, N.LAYOUT_CELL_PAIR) if unsteady else (N.LAYOUT_SINGLE, N.LAYOUT_CELL)
+    node = _field(vels, pts, (n, n, n), lay0, periodic=per, dt_snap=0.25)
+    cell = _field(vels, pts, (n, n, n), lay1, periodic=per, dt_snap=0.25)
+    x0 = H.seeds_3d(9, lo=0.05, hi=6.2)  # groups across the periodic seams
+    kw = dict(method="rk4", time_mode="linear" if unsteady else "steady", n_steps=80, h=0.0125,
+              t_start=0.05, map_period=1.0)
+    f0, p0 = _run(node, x0, **kw)
+    f1, p1 = _run(cell, x0, **kw)
+    assert torch.equal(p1, p0)
+    assert torch.equal(f1, f0)
+    assert np.isfinite(f0.cpu().numpy()).all()
+
+
+def test_cell_pair_batch_matches_node_layout():
+    from pyftle_amd import _native as N
+    from pyftle_amd.batch import BatchPlan, Window
+
+    specs = [((32, 16), 9, (7, 3), 12, 0.05, 0.0), ((48, 24), 13, (21, 9), 25, -0.05, 0.6)]
+    outs = {}
+    for lay in (N.LAYOUT_PAIR, N.LAYOUT_CELL_PAIR):
+        wins = []
+        for grid, n_snap, (sx, sy), steps, h, t0 in specs:
+            vels, pts = H.double_gyre_window(nx=grid[0], ny=grid[1], n_snap=n_snap)
+            f = _field(vels, pts, grid, lay)
+            pos = torch.as_tensor(H.seeds_2d(sx, sy), device="cuda")
+            wins.append(Window(field=f, positions=pos, n_steps=steps, h=h, t_start=t0, map_period=steps * abs(h)))
+        plan = BatchPlan(wins, method="rk4", time_mode="linear")
+        plan.run()
+        torch.cuda.synchronize()
+        outs[lay] = [w.ftle.clone() for w in wins]
+        plan.close()
+    for a, b in zip(outs[N.LAYOUT_PAIR], outs[N.LAYOUT_CELL_PAIR]):
+        assert torch.equal(a, b)
+
+
+def test_cell_layouts_refuse_other_uses():
+    from pyftle_amd import _native as N
+    from pyftle_amd import window as W
+
+    vels, pts = H.double_gyre_window(nx=16, ny=8, n_snap=2)
+    with pytest.raises(RuntimeError, match="CELL layouts hold FP32"):
+        _field(vels, pts, (16, 8), N.LAYOUT_CELL, precision="float64")
+    f = _field(vels, pts, (16, 8), N.LAYOUT_CELL)
+    x0 = torch.as_tensor(H.seeds_2d(3, 2), device="cuda")
+    with pytest.raises(RuntimeError, match="compute must be NATIVE"):
+        W.advect_ftle(f, x0, method="rk4", time_mode="steady", n_steps=2, h=0.1, map_period=0.2,
+                      cfg=W.LaunchConfig(compute=N.COMPUTE_MIXED))
+    with pytest.raises(RuntimeError, match="LINEAR time needs"):
+        W.advect_ftle(f, x0, method="rk4", time_mode="linear", n_steps=2, h=0.1, map_period=0.2)
+    with pytest.raises(RuntimeError, match="FP32 advection kernels only"):
+        f.sample(torch.zeros(4, 2, device="cuda", dtype=torch.float64))
