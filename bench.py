@@ -337,17 +337,17 @@ def run_surrogate(torch, wl, args):
     plan.close()
 
 
-def _abc_unsteady_field_on_device(torch, n, t_first, n_slots, dt_snap):
+def _abc_unsteady_field_on_device(torch, n, t_first, n_slots, dt_snap, cell=False):
     """Unsteady ABC flow, A(t) = sqrt(3) + 0.5 sin(pi t / 5), B = sqrt(2), C = 1, on a
-    periodic n^3 grid over [0, 2 pi)^3 (no endpoint), FP32 PAIR snapshots at
-    t_first + k dt_snap, k < n_slots."""
+    periodic n^3 grid over [0, 2 pi)^3 (no endpoint), FP32 time-pair snapshots (node or
+    cell records) at t_first + k dt_snap, k < n_slots."""
     from pyftle_amd import _native as N
     from pyftle_amd.field import DeviceField, GridGeometry
 
     L = 2.0 * math.pi
     geom = GridGeometry.uniform((n, n, n), (0.0,) * 3, (L / n,) * 3, periodic=(True,) * 3)
-    field = DeviceField(geom, dtype="float32", layout=N.LAYOUT_PAIR, ring_slots=n_slots, n_snapshots=n_slots,
-                        t0=t_first, dt_snap=dt_snap)
+    field = DeviceField(geom, dtype="float32", layout=N.LAYOUT_CELL_PAIR if cell else N.LAYOUT_PAIR,
+                        ring_slots=n_slots, n_snapshots=n_slots, t0=t_first, dt_snap=dt_snap)
     ax = torch.arange(n, dtype=torch.float64, device="cuda") * (L / n)
     Z, Y, X = torch.meshgrid(ax, ax, ax, indexing="ij")  # flat ix + n*(iy + n*iz)
     B, C = math.sqrt(2.0), 1.0
@@ -446,8 +446,23 @@ def run_surrogate3d(torch, wl, args):
         return out, a.elapsed_time(b) / reps
 
     n_seeds = seeds.shape[0]
-    _, rk4_ms = timed(lambda: sf.rk4_ftle(seeds, wl["t0s"][0]))
-    log(f"plain RK4, all {n_seeds} seeds, one window: {rk4_ms:.1f} ms")
+    _, rk4_node_ms = timed(lambda: sf.rk4_ftle(seeds, wl["t0s"][0]))
+    # the fastest plain RK4 of this build: cell-pair records, in a ring holding the snapshots
+    # of the timed window only (the whole stored range would take 160 GB as cell pairs)
+    dts, t_first = 10.0 / 64.0, -1.25
+    k0 = int(math.floor((wl["t0s"][0] - t_first) / dts))
+    n_win = int(math.ceil(wl["T"] / dts)) + 2
+    cell_field = _abc_unsteady_field_on_device(torch, wl["config"]["grid"][0], t_first + k0 * dts, n_win, dts,
+                                               cell=True)
+    sf_cell = SF.SurrogateFTLE(ens, cell_field, wl["T"], wl["h"], tol=args.surrogate_tol, chunk=1 << 19)
+    ref_cell, rk4_ms = timed(lambda: sf_cell.rk4_ftle(seeds, wl["t0s"][0]))
+    ref_node = sf.rk4_ftle(seeds, wl["t0s"][0])
+    # the two rings start at different times, so the time weights round differently
+    layout_diff = ((ref_cell - ref_node).abs().max() / (ref_node.max() - ref_node.min())).item()
+    del sf_cell, cell_field, ref_cell, ref_node
+    torch.cuda.empty_cache()
+    log(f"plain RK4, all {n_seeds} seeds, one window: {rk4_ms:.1f} ms with cell-pair records "
+        f"({rk4_node_ms:.1f} ms node records; max FTLE difference / range {layout_diff:.1e})")
     per_t0 = []
     for t0 in wl["t0s"]:
         ref = sf.rk4_ftle(judge_seeds, t0)
@@ -487,7 +502,9 @@ def run_surrogate3d(torch, wl, args):
                      "frac": rows4 * fpq / (jm4 * 1e-3) / 1e12 / 2500.0, "flops_per_query": fpq,
                      "kernel": "fused_mlp_kernel<512,4,3,3> and <512,4,0,3>"},
             "train_s": t_train, "data_s": t_data},
-        "rk4": {"ms_all_seeds_one_window": rk4_ms, "seeds_per_s": n_seeds / (rk4_ms * 1e-3)},
+        "rk4": {"ms_all_seeds_one_window": rk4_ms, "seeds_per_s": n_seeds / (rk4_ms * 1e-3),
+                "records": "cell pairs (ring of the timed window's snapshots)",
+                "ms_node_records": rk4_node_ms, "max_ftle_diff_over_range_node_vs_cell": layout_diff},
         "speedup_vs_gpu_rk4": rk4_ms / sur_ms,
         "surrogate_only_speedup_vs_gpu_rk4": {"jac_all": rk4_ms / jm4, "jac_1": rk4_ms / mean("jac1", "ms")},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(_abc_cpu_workload(torch, wl, seeds), budget_s=15.0),
