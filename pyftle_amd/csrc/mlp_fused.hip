@@ -8,9 +8,10 @@
 // activations of its rows in LDS for the whole network, and only the weights move: from L2
 // straight into registers.
 //
-// Formulation: out^T = W in^T per layer with v_mfma_f32_16x16x32_bf16.  The weights are
-// the A operand (16 output features x 32 inputs per fragment), the activations the B
-// operand (32 inputs x 16 rows) read from LDS.  Wave w of 4 owns output features
+// Formulation: out^T = A in^T per layer (A the layer's weights) with
+// v_mfma_f32_16x16x32_bf16.  The weights are the MFMA A operand (16 output features x 32
+// inputs per fragment), the activations the B operand (32 inputs x 16 rows) read from
+// LDS.  Wave w of 4 owns output features
 // W/4 w .. W/4 (w + 1) - 1 (W/64 tiles of 16) for all 16 RT rows of the work-group (RT
 // row tiles of 16), so each weight fragment feeds RT MFMAs and each activation fragment
 // W/64.  The weights are packed fragment by fragment (pyftle_mlp_pack_fused): the 64 lanes'
@@ -25,8 +26,8 @@
 // next layer's first ones are issued during the last k-step), a barrier, the epilogue
 // SiLU(z + b) written back to LDS in place as bf16 at raised wave priority, a barrier.
 // Two work-groups share a CU (57-75 KB of LDS each), so one's epilogue runs under the
-// other's MFMAs.  The output layer
-// (O <= 3) is FP32 on the last hidden layer's registers, reduced over lanes and waves.
+// other's MFMAs.  The output layer (O <= 3) is FP32 on the last hidden layer's registers,
+// reduced over lanes and waves.
 
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
