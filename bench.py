@@ -1056,7 +1056,7 @@ def main():
         return run_surrogate3d_infer(torch, wl, args)
     if wl["name"] == "surrogate3d":
         return run_surrogate3d(torch, wl, args)
-    cfg = best_config(shape_class(wl["dim"], wl["n_seeds"], wl["time_mode"]))
+    cfg = best_config(shape_class(wl["dim"], wl["n_seeds"], wl["time_mode"], records=_records_name(wl["dim"])))
     if args.groups_per_block > 0:
         cfg.groups_per_block = args.groups_per_block
     if args.lane_order >= 0:

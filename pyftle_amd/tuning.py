@@ -22,10 +22,12 @@ CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_cache.j
 _lock = threading.Lock()
 
 
-def shape_class(dim: int, n_seeds: int, time_mode: str, precision: str = "f32") -> str:
-    """Bucket by dimension, time treatment, precision and log2 of the seed count."""
+def shape_class(dim: int, n_seeds: int, time_mode: str, precision: str = "f32", records: str = "node") -> str:
+    """Bucket by dimension, time treatment, precision, log2 of the seed count and, for the
+    per-cell record layouts, the layout (their best launch shape differs)."""
     b = max(0, int(n_seeds).bit_length() - 1)
-    return f"d{dim}-{time_mode}-{precision}-s{b}"
+    key = f"d{dim}-{time_mode}-{precision}-s{b}"
+    return key + "-cell" if records == "cell" else key
 
 
 def _read():

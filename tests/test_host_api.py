@@ -234,6 +234,7 @@ def test_tuning_shape_classes():
     from pyftle_amd import tuning
 
     assert tuning.shape_class(3, 128**3, "steady") == "d3-steady-f32-s21"
+    assert tuning.shape_class(3, 128**3, "steady", records="cell") == "d3-steady-f32-s21-cell"
     cands = tuning.candidates(3)
     assert all(c.groups_per_block * 6 <= 512 for c in cands)
     assert tuning.best_config("no-such-class").groups_per_block == 64
