@@ -22,10 +22,12 @@ from dataclasses import dataclass
 
 import torch
 
+from . import _native as N
 from . import window as W
 from .batch import BatchConfig, BatchPlan, Window
 from .field import DeviceField
 from .surrogate import FlowMapEnsemble, FlowMapMLP, MLPSpec, ftle_from_displacement_jacobian, train
+from .tuning import best_config, shape_class
 
 
 def uniform_seeds(n: int, lower, upper, gen: torch.Generator, device="cuda") -> torch.Tensor:
@@ -111,11 +113,15 @@ class SurrogateFTLE:
         return torch.cat(out_f), torch.cat(out_s)
 
     def rk4_ftle(self, seeds: torch.Tensor, t0: float) -> torch.Tensor:
-        """Plain RK4 FTLE of the same seeds (neighbour groups at +-delta)."""
+        """Plain RK4 FTLE of the same seeds (neighbour groups at +-delta), with the launch
+        shape tuned for this field's shape class when the tuning cache holds one."""
         pos = neighbour_groups(seeds, self.delta)
         steps = int(round(self.T / abs(self.h)))
+        cell = self.field.layout in (N.LAYOUT_CELL, N.LAYOUT_CELL_PAIR)
+        cfg = best_config(shape_class(seeds.shape[1], seeds.shape[0], self.time_mode,
+                                      records="cell" if cell else "node"))
         return W.advect_ftle(self.field, pos, method="rk4", time_mode=self.time_mode, n_steps=steps,
-                             h=self.h, t_start=t0, map_period=steps * abs(self.h),
+                             h=self.h, t_start=t0, map_period=steps * abs(self.h), cfg=cfg,
                              order="morton" if seeds.shape[0] >= 1 << 18 else None)
 
     def ftle(self, seeds: torch.Tensor, t0: float) -> SurrogateResult:
