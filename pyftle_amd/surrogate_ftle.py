@@ -22,12 +22,10 @@ from dataclasses import dataclass
 
 import torch
 
-from . import _native as N
 from . import window as W
 from .batch import BatchConfig, BatchPlan, Window
 from .field import DeviceField
 from .surrogate import FlowMapEnsemble, FlowMapMLP, MLPSpec, ftle_from_displacement_jacobian, train
-from .tuning import best_config, shape_class
 
 
 def uniform_seeds(n: int, lower, upper, gen: torch.Generator, device="cuda") -> torch.Tensor:
@@ -117,11 +115,8 @@ class SurrogateFTLE:
         shape tuned for this field's shape class when the tuning cache holds one."""
         pos = neighbour_groups(seeds, self.delta)
         steps = int(round(self.T / abs(self.h)))
-        cell = self.field.layout in (N.LAYOUT_CELL, N.LAYOUT_CELL_PAIR)
-        cfg = best_config(shape_class(seeds.shape[1], seeds.shape[0], self.time_mode,
-                                      records="cell" if cell else "node"))
         return W.advect_ftle(self.field, pos, method="rk4", time_mode=self.time_mode, n_steps=steps,
-                             h=self.h, t_start=t0, map_period=steps * abs(self.h), cfg=cfg,
+                             h=self.h, t_start=t0, map_period=steps * abs(self.h),
                              order="morton" if seeds.shape[0] >= 1 << 18 else None)
 
     def ftle(self, seeds: torch.Tensor, t0: float) -> SurrogateResult:

@@ -30,14 +30,23 @@ def shape_class(dim: int, n_seeds: int, time_mode: str, precision: str = "f32", 
     return key + "-cell" if records == "cell" else key
 
 
+_memo = {"mtime": None, "data": {}}
+
+
 def _read():
-    if os.path.exists(CACHE):
+    """The cache file's entries (re-read only when the file changed)."""
+    try:
+        mtime = os.path.getmtime(CACHE)
+    except OSError:
+        return {}
+    if _memo["mtime"] != mtime:
         try:
             with open(CACHE) as f:
-                return json.load(f)
+                data = json.load(f)
         except (OSError, ValueError):
-            return {}
-    return {}
+            data = {}
+        _memo["mtime"], _memo["data"] = mtime, data
+    return dict(_memo["data"])
 
 
 def best_config(key: str) -> LaunchConfig:
@@ -89,6 +98,7 @@ def autotune(run, dim: int, key: str, repeats: int = 3, save: bool = True):
                          "xcd_remap": best.xcd_remap, "ms": table[0][0]}
             with open(CACHE, "w") as f:
                 json.dump(data, f, indent=1, sort_keys=True)
+            _memo["mtime"] = None
     return best, [(ms, vars(c)) for ms, c in table]
 
 
@@ -112,3 +122,4 @@ def save_batch_config(key: str, cfg, ms: float) -> None:
         data[key] = dict(vars(cfg), ms=ms)
         with open(CACHE, "w") as f:
             json.dump(data, f, indent=1, sort_keys=True)
+        _memo["mtime"] = None

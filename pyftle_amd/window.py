@@ -77,6 +77,18 @@ def make_opts(method, time_mode, n_steps, h, first=0, t_start=0.0, have_prev=Fal
     return o
 
 
+def tuned_config(field: DeviceField, n_groups: int, time_mode) -> LaunchConfig:
+    """Launch shape from the tuning cache for this field's shape class (dimension, time
+    treatment, precision, seed count, record layout), the defaults when it was never tuned.
+    The shape changes only the speed: every seed group is independent."""
+    from .tuning import best_config, shape_class
+
+    tm = time_mode if isinstance(time_mode, str) else {v: k for k, v in TIME_MODES.items()}.get(time_mode, "")
+    prec = "f32" if field.dtype == torch.float32 else "f64"
+    cell = field.layout in (N.LAYOUT_CELL, N.LAYOUT_CELL_PAIR)
+    return best_config(shape_class(field.geom.dim, n_groups, tm, prec, records="cell" if cell else "node"))
+
+
 def advect_ftle(field: DeviceField, positions: torch.Tensor, *, method="rk4", time_mode="frozen",
                 n_steps: int, h: float, map_period: float, first: int = 0, t_start: float = 0.0,
                 positions_out: torch.Tensor | None = None, ftle_out: torch.Tensor | None = None,
@@ -97,6 +109,8 @@ def advect_ftle(field: DeviceField, positions: torch.Tensor, *, method="rk4", ti
     n_groups = positions.shape[0] // nb
     if ftle_out is None:
         ftle_out = torch.empty(n_groups, dtype=torch.float64, device=positions.device)
+    if cfg is None:
+        cfg = tuned_config(field, n_groups, time_mode)
     opts = make_opts(method, time_mode, n_steps, h, first, t_start, False, cfg)
     if isinstance(order, str):
         if order != "morton":
