@@ -302,7 +302,7 @@ def run_surrogate(torch, wl, args):
     sweep = []
     for tol in sorted({args.surrogate_tol, 1e-3, 3e-3, 1e-2, 3e-2}):
         sf.tol = tol
-        res, ms = timed(lambda: [sf.ftle(seeds, t0) for t0 in t0s], 1)
+        res, ms = timed(lambda: sf.ftle_windows(seeds, t0s), 1)
         rm = [SF.relative_rmse(r.ftle, w.ftle) for r, w in zip(res, wins)]
         fb = sum(int(r.fallback.sum().item()) for r in res) / n_sw
         sweep.append({"tol": tol, "ms": ms, "seeds_per_s": n_sw / (ms * 1e-3), "fallback_fraction": fb,

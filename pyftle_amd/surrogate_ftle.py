@@ -127,6 +127,30 @@ class SurrogateFTLE:
             f[idx] = self.rk4_ftle(seeds[idx].contiguous(), t0)
         return SurrogateResult(ftle=f, fallback=bad, spread=spread)
 
+    def ftle_windows(self, seeds: torch.Tensor, t0s, cfg: BatchConfig | None = None) -> list[SurrogateResult]:
+        """``ftle`` for several window starts: the ensemble per start, then the fallback seeds
+        of every start in one batched persistent RK4 launch (one window per start) instead of
+        one small launch each.  The batched kernel is bit-equal to single launches."""
+        steps = int(round(self.T / abs(self.h)))
+        results, wins, where = [], [], []
+        for t0 in t0s:
+            f, spread = self.predict(seeds, t0)
+            bad = spread > self.tol
+            results.append(SurrogateResult(ftle=f, fallback=bad, spread=spread))
+            idx = torch.nonzero(bad).flatten()
+            if idx.numel():
+                pos = neighbour_groups(seeds[idx].contiguous(), self.delta)
+                wins.append(Window(field=self.field, positions=pos, n_steps=steps, h=self.h, t_start=float(t0),
+                                   map_period=steps * abs(self.h)))
+                where.append((len(results) - 1, idx))
+        if wins:
+            plan = BatchPlan(wins, method="rk4", time_mode=self.time_mode, cfg=cfg)
+            plan.run()
+            for w, (k, idx) in zip(wins, where):
+                results[k].ftle[idx] = w.ftle
+            plan.close()  # waits for the device
+        return results
+
 
 def neighbour_groups(seeds: torch.Tensor, delta: float) -> torch.Tensor:
     """Block-ordered [L, R, T, B(, F, Bk)] neighbours (particles.py:170,191,212)."""

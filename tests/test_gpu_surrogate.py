@@ -89,6 +89,15 @@ def test_fallback_recomputes_disagreeing_seeds_with_rk4(gyre_setup):
         res = sf.ftle(seeds, 0.7)
         assert bool(res.fallback.all())
         assert torch.equal(res.ftle, sf.rk4_ftle(seeds, 0.7))
+        # several window starts, fallback seeds batched into one launch: the same fields
+        batched = sf.ftle_windows(seeds, [0.7, 1.1, 0.2])
+        for t0, r in zip([0.7, 1.1, 0.2], batched):
+            assert bool(r.fallback.all())
+            assert torch.equal(r.ftle, sf.ftle(seeds, t0).ftle)
+        sf.tol = 1e9  # no fallback at all
+        for t0, r in zip([0.4, 1.3], sf.ftle_windows(seeds, [0.4, 1.3])):
+            assert not bool(r.fallback.any())
+            assert torch.equal(r.ftle, sf.ftle(seeds, t0).ftle)
     finally:
         bad.params.copy_(saved)
         bad.sync_bf16()
