@@ -105,6 +105,32 @@ def test_cell_layouts_3d_periodic_bit_identical(unsteady):
     assert np.isfinite(f0.cpu().numpy()).all()
 
 
+@pytest.mark.parametrize("method", ["euler", "ab2", "rk4"])
+def test_cell_layout_plain_advection_bit_identical(method):
+    """pyftle_advect (the per-step integrator entry point, no FTLE) on 2D periodic and 3D
+    bounded fields, FP64 positions."""
+    from pyftle_amd import _native as N
+    from pyftle_amd import window as W
+
+    cases = []
+    vels, pts = H.double_gyre_window(nx=40, ny=20, n_snap=3)
+    cases.append((vels, pts, (40, 20), (True, False), H.seeds_2d(17, 9)))
+    vels3, pts3 = H.abc_window(n=12, endpoint=True)
+    cases.append((vels3, pts3, (12, 12, 12), None, H.seeds_3d(5, lo=0.3, hi=6.0)))
+    for vels_, pts_, shape, per, x0 in cases:
+        lay0, lay1 = N.LAYOUT_SINGLE, N.LAYOUT_CELL
+        out = []
+        for lay in (lay0, lay1):
+            f = _field(vels_, pts_, shape, lay, periodic=per)
+            pos = torch.as_tensor(x0, device="cuda").clone()
+            vprev = torch.zeros_like(pos) if method == "ab2" else None
+            for k in range(2):  # two calls: AB2 carries its previous velocity across them
+                W.advect(f, pos, method=method, time_mode="steady", n_steps=7, h=0.03, first=k % len(vels_),
+                         vprev=vprev, have_prev=(method == "ab2" and k > 0))
+            out.append(pos)
+        assert torch.equal(out[1], out[0])
+
+
 def test_cell_pair_batch_matches_node_layout():
     from pyftle_amd import _native as N
     from pyftle_amd.batch import BatchPlan, Window
