@@ -390,9 +390,15 @@ __device__ __forceinline__ int cell_axis(const FieldDev &f, int a, int cell, flo
   return i;
 }
 
-template <int DIM, bool PAIR, bool CELL>
+// With CACHE (steady single-level per-cell records) the lane keeps the last
+// record it read and its record index in (ckey, crec): a sample in the same
+// cell as the previous one reads no memory, and only the lanes that moved to
+// another cell issue the loads.  A record's contents depend on its index alone
+// (one snapshot for the whole launch), so the result is the same bit for bit.
+template <int DIM, bool PAIR, bool CELL, bool CACHE = false>
 __device__ __forceinline__ void sample_cell(const FieldDev &f, __amdgpu_buffer_rsrc_t rs, float w,
-                                            const int *cell, const float *t, float *vel) {
+                                            const int *cell, const float *t, float *vel,
+                                            int *ckey = nullptr, float *crec = nullptr) {
   constexpr int C = RecInfo<DIM, PAIR>::comps;
   constexpr int L = RecInfo<DIM, PAIR>::per_level;
   constexpr int NF = C > 4 ? 4 : C;
@@ -413,10 +419,14 @@ __device__ __forceinline__ void sample_cell(const FieldDev &f, __amdgpu_buffer_r
     // one record per cell: level l, corner q, component comp at float (l NQ + q) DIM + comp
     constexpr int LV = PAIR ? 2 : 1;
     constexpr int RF = NQ * DIM * LV;  // 8, 16, 24 or 48 floats
-    float rec[RF];
+    float local[RF];
+    float *rec = CACHE ? crec : local;
     const unsigned roff = (unsigned)base * (unsigned)(RF * 4);  // out-of-range offsets read 0
+    if (!CACHE || base != *ckey) {
 #pragma unroll
-    for (int h = 0; h < RF / 4; ++h) BufVec<4>::load(rs, (int)(roff + 16u * h), &rec[h * 4]);
+      for (int h = 0; h < RF / 4; ++h) BufVec<4>::load(rs, (int)(roff + 16u * h), &rec[h * 4]);
+      if constexpr (CACHE) *ckey = base;
+    }
 #pragma unroll
     for (int l = 0; l < LV; ++l)
 #pragma unroll
@@ -488,7 +498,6 @@ __device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t l
   const int64_t pidx = (int64_t)j * a.n_groups + (active ? seed_at(a, s) : s);
   const FieldDev &f = a.f;
 
-  double x0[DIM];
   int cell[DIM], wraps[DIM];
   float frac[DIM], comp[DIM];
   float sh[DIM], sf[DIM];
@@ -499,7 +508,6 @@ __device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t l
     if (active)
       v = a.pos_f64 ? ((const double *)a.pos_in)[pidx * DIM + c]
                     : (double)((const float *)a.pos_in)[pidx * DIM + c];
-    x0[c] = v;
     const double fi = (v - f.origin[c]) * f.inv_spacing[c];
     double fl = floor(fi);
     fl = fl < -1.0e9 ? -1.0e9 : (fl > 1.0e9 ? 1.0e9 : fl);
@@ -537,6 +545,14 @@ __device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t l
 
   const int bytes = (int)f.slot_stride;
   const __amdgpu_buffer_rsrc_t steady = make_rsrc(slot_ptr(f, a.first), bytes);
+  // per-lane record cache (steady per-cell records): key -1 is byte offset
+  // 2^32 - record size, past the end of any slot, whose load reads zeros
+  constexpr bool CACHE = CELL && !PAIR && TMODE == PYFTLE_TIME_STEADY;
+  constexpr int CRF = CACHE ? ((DIM == 2) ? 8 : 24) : 1;
+  int ckey = -1;
+  float crec[CRF];
+#pragma unroll
+  for (int i = 0; i < CRF; ++i) crec[i] = 0.0f;
   for (int step = 0; step < a.n_steps; ++step) {
     __amdgpu_buffer_rsrc_t r0 = steady, r1 = steady, r2 = steady;
     double w0 = 0.0, w1 = 0.0, w2 = 0.0;
@@ -558,18 +574,18 @@ __device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t l
     double inc[DIM];
 #pragma unroll
     for (int c = 0; c < DIM; ++c) tp[c] = frac[c];
-    sample_cell<DIM, PAIR, CELL>(f, r0, (float)w0, cell, tp, k1);
+    sample_cell<DIM, PAIR, CELL, CACHE>(f, r0, (float)w0, cell, tp, k1, &ckey, crec);
     if (METHOD == PYFTLE_RK4) {
       float k2[DIM], k3[DIM], k4[DIM];
 #pragma unroll
       for (int c = 0; c < DIM; ++c) tp[c] = fmaf(sh[c], k1[c], frac[c]);
-      sample_cell<DIM, PAIR, CELL>(f, r1, (float)w1, cell, tp, k2);
+      sample_cell<DIM, PAIR, CELL, CACHE>(f, r1, (float)w1, cell, tp, k2, &ckey, crec);
 #pragma unroll
       for (int c = 0; c < DIM; ++c) tp[c] = fmaf(sh[c], k2[c], frac[c]);
-      sample_cell<DIM, PAIR, CELL>(f, r1, (float)w1, cell, tp, k3);
+      sample_cell<DIM, PAIR, CELL, CACHE>(f, r1, (float)w1, cell, tp, k3, &ckey, crec);
 #pragma unroll
       for (int c = 0; c < DIM; ++c) tp[c] = fmaf(sf[c], k3[c], frac[c]);
-      sample_cell<DIM, PAIR, CELL>(f, r2, (float)w2, cell, tp, k4);
+      sample_cell<DIM, PAIR, CELL, CACHE>(f, r2, (float)w2, cell, tp, k4, &ckey, crec);
 #pragma unroll
       for (int c = 0; c < DIM; ++c)
         inc[c] = s6d[c] * ((double)k1[c] + 2.0 * ((double)k2[c] + (double)k3[c]) + (double)k4[c]);
@@ -613,6 +629,23 @@ __device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t l
     const double idx = (double)cell[c] + (double)wraps[c] * (double)f.n[c] + (double)frac[c] -
                        (double)comp[c];
     fin[c] = f.origin[c] + idx * f.spacing[c];
+  }
+  // The seed position is read again for the epilogue instead of being held
+  // through the step loop (six registers of FP64).  The empty asm hides that
+  // the index equals the first read's, so that read is not reused; pos_out,
+  // which may be pos_in, is written after this.
+  double x0[DIM];
+  if constexpr (FTLE) {
+    int64_t q = pidx;
+    asm volatile("" : "+v"(q));
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+      double v = 0.0;
+      if (active)
+        v = a.pos_f64 ? ((const double *)a.pos_in)[q * DIM + c]
+                      : (double)((const float *)a.pos_in)[q * DIM + c];
+      x0[c] = v;
+    }
   }
   if (active && a.pos_out) {
 #pragma unroll
