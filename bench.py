@@ -43,20 +43,21 @@ def log(msg):
 
 # record layout of the FP32 fields: "node" (one record per grid node, the layout of the
 # reference's arrays), "cell" (one record per cell holding its 2^dim corners) or "auto"
-# (cell in 3D: 6 contiguous 16-byte loads per trilinear sample instead of 8 scattered
-# ones, abc128 226 -> 187 ms; node in 2D, where a cell record is as many loads and the
-# 4x larger slot costs cache hits: gyre512 1.22 -> 1.42 ms, batch64 536 -> 601 ms)
+# (cell records: a sample is one record, 6 contiguous 16-byte loads in 3D instead of 8
+# scattered node records, and the kernels keep each lane's last record in registers, so
+# a sample in the same cell as the lane's previous one loads nothing; abc128 226 -> 118 ms,
+# gyre512 1.23 -> 0.94 ms, batch64 543 -> 474 ms against node records)
 RECORDS = "auto"
 
 
 def _layout(N, pair, dim):
-    if RECORDS == "cell" or (RECORDS == "auto" and dim == 3):
+    if RECORDS in ("cell", "auto"):
         return N.LAYOUT_CELL_PAIR if pair else N.LAYOUT_CELL
     return N.LAYOUT_PAIR if pair else N.LAYOUT_SINGLE
 
 
 def _records_name(dim):
-    return "cell" if RECORDS == "cell" or (RECORDS == "auto" and dim == 3) else "node"
+    return "cell" if RECORDS in ("cell", "auto") else "node"
 
 
 def build_abc128(torch, n=128, ns=128, delta=1e-3, h=0.01, steps=1000):
@@ -1004,7 +1005,7 @@ def main():
     ap.add_argument("--records", default="auto", choices=["auto", "node", "cell"],
                     help="record layout of the FP32 fields of abc128, gyre512, batch64 and the 2D surrogate "
                          "workloads: one record per grid node, one per cell holding its corners, or auto "
-                         "(cell in 3D, node in 2D)")
+                         "(cell records)")
     ap.add_argument("--groups-per-block", type=int, default=0)
     ap.add_argument("--lane-order", type=int, default=-1)
     ap.add_argument("--xcd-remap", type=int, default=-1)

@@ -390,15 +390,18 @@ __device__ __forceinline__ int cell_axis(const FieldDev &f, int a, int cell, flo
   return i;
 }
 
-// With CACHE (steady single-level per-cell records) the lane keeps the last
-// record it read and its record index in (ckey, crec): a sample in the same
-// cell as the previous one reads no memory, and only the lanes that moved to
-// another cell issue the loads.  A record's contents depend on its index alone
-// (one snapshot for the whole launch), so the result is the same bit for bit.
-template <int DIM, bool PAIR, bool CELL, bool CACHE = false>
+// Per-lane cache of the last per-cell record a lane read: its record index,
+// its time slot and its floats.  A sample of the same record in the same slot
+// reads no memory; only the lanes whose sample moved to another cell or slot
+// take part in the loads, and a wave in which none did skips them.  Within a
+// launch a record's contents depend on (slot, index) alone, so the results are
+// the same bit for bit.  Steady fields pass slot 0 throughout (a constant the
+// compiler folds away); the others start from slot -1, which no sample has.
+template <int DIM, bool PAIR, bool CELL, bool CACHE = false, bool SLOTS = false>
 __device__ __forceinline__ void sample_cell(const FieldDev &f, __amdgpu_buffer_rsrc_t rs, float w,
                                             const int *cell, const float *t, float *vel,
-                                            int *ckey = nullptr, float *crec = nullptr) {
+                                            int *ckey = nullptr, int *cslot = nullptr,
+                                            float *crec = nullptr, int slot = 0) {
   constexpr int C = RecInfo<DIM, PAIR>::comps;
   constexpr int L = RecInfo<DIM, PAIR>::per_level;
   constexpr int NF = C > 4 ? 4 : C;
@@ -422,11 +425,12 @@ __device__ __forceinline__ void sample_cell(const FieldDev &f, __amdgpu_buffer_r
     float local[RF];
     float *rec = CACHE ? crec : local;
     const unsigned roff = (unsigned)base * (unsigned)(RF * 4);  // out-of-range offsets read 0
-    if (!CACHE || base != *ckey) {
+    if (!CACHE || base != *ckey || (SLOTS && slot != *cslot)) {
 #pragma unroll
       for (int h = 0; h < RF / 4; ++h) BufVec<4>::load(rs, (int)(roff + 16u * h), &rec[h * 4]);
       if constexpr (CACHE) *ckey = base;
     }
+    if constexpr (CACHE && SLOTS) *cslot = slot;
 #pragma unroll
     for (int l = 0; l < LV; ++l)
 #pragma unroll
@@ -545,47 +549,48 @@ __device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t l
 
   const int bytes = (int)f.slot_stride;
   const __amdgpu_buffer_rsrc_t steady = make_rsrc(slot_ptr(f, a.first), bytes);
-  // per-lane record cache (steady per-cell records): key -1 is byte offset
-  // 2^32 - record size, past the end of any slot, whose load reads zeros
-  constexpr bool CACHE = CELL && !PAIR && TMODE == PYFTLE_TIME_STEADY;
-  constexpr int CRF = CACHE ? ((DIM == 2) ? 8 : 24) : 1;
-  int ckey = -1;
+  // per-lane record cache for per-cell records (key -1: byte offset 2^32 minus
+  // the record size, past the end of any slot, whose load reads zeros)
+  constexpr bool CACHE = CELL;
+  constexpr int CRF = CACHE ? (1 << DIM) * DIM * (PAIR ? 2 : 1) : 1;
+  int ckey = -1, cslot = -1;
   float crec[CRF];
 #pragma unroll
   for (int i = 0; i < CRF; ++i) crec[i] = 0.0f;
   for (int step = 0; step < a.n_steps; ++step) {
     __amdgpu_buffer_rsrc_t r0 = steady, r1 = steady, r2 = steady;
     double w0 = 0.0, w1 = 0.0, w2 = 0.0;
+    int q0 = 0, q1 = 0, q2 = 0;  // time slots of the three sample times (cache keys)
     if (TMODE == PYFTLE_TIME_FROZEN) {
-      r0 = r1 = r2 = make_rsrc(slot_ptr(f, a.first + step), bytes);
+      q0 = q1 = q2 = a.first + step;
+      r0 = r1 = r2 = make_rsrc(slot_ptr(f, q0), bytes);
     } else if (TMODE == PYFTLE_TIME_LINEAR) {
       const double tt = a.t_start + step * a.h;
-      int k;
-      time_linear_lookup(f, tt, k, w0);
-      r0 = make_rsrc(slot_ptr(f, k), bytes);
+      time_linear_lookup(f, tt, q0, w0);
+      r0 = make_rsrc(slot_ptr(f, q0), bytes);
       if (METHOD == PYFTLE_RK4) {
-        time_linear_lookup(f, tt + 0.5 * a.h, k, w1);
-        r1 = make_rsrc(slot_ptr(f, k), bytes);
-        time_linear_lookup(f, tt + a.h, k, w2);
-        r2 = make_rsrc(slot_ptr(f, k), bytes);
+        time_linear_lookup(f, tt + 0.5 * a.h, q1, w1);
+        r1 = make_rsrc(slot_ptr(f, q1), bytes);
+        time_linear_lookup(f, tt + a.h, q2, w2);
+        r2 = make_rsrc(slot_ptr(f, q2), bytes);
       }
     }
     float tp[DIM], k1[DIM];
     double inc[DIM];
 #pragma unroll
     for (int c = 0; c < DIM; ++c) tp[c] = frac[c];
-    sample_cell<DIM, PAIR, CELL, CACHE>(f, r0, (float)w0, cell, tp, k1, &ckey, crec);
+    sample_cell<DIM, PAIR, CELL, CACHE, TMODE != PYFTLE_TIME_STEADY>(f, r0, (float)w0, cell, tp, k1, &ckey, &cslot, crec, q0);
     if (METHOD == PYFTLE_RK4) {
       float k2[DIM], k3[DIM], k4[DIM];
 #pragma unroll
       for (int c = 0; c < DIM; ++c) tp[c] = fmaf(sh[c], k1[c], frac[c]);
-      sample_cell<DIM, PAIR, CELL, CACHE>(f, r1, (float)w1, cell, tp, k2, &ckey, crec);
+      sample_cell<DIM, PAIR, CELL, CACHE, TMODE != PYFTLE_TIME_STEADY>(f, r1, (float)w1, cell, tp, k2, &ckey, &cslot, crec, q1);
 #pragma unroll
       for (int c = 0; c < DIM; ++c) tp[c] = fmaf(sh[c], k2[c], frac[c]);
-      sample_cell<DIM, PAIR, CELL, CACHE>(f, r1, (float)w1, cell, tp, k3, &ckey, crec);
+      sample_cell<DIM, PAIR, CELL, CACHE, TMODE != PYFTLE_TIME_STEADY>(f, r1, (float)w1, cell, tp, k3, &ckey, &cslot, crec, q1);
 #pragma unroll
       for (int c = 0; c < DIM; ++c) tp[c] = fmaf(sf[c], k3[c], frac[c]);
-      sample_cell<DIM, PAIR, CELL, CACHE>(f, r2, (float)w2, cell, tp, k4, &ckey, crec);
+      sample_cell<DIM, PAIR, CELL, CACHE, TMODE != PYFTLE_TIME_STEADY>(f, r2, (float)w2, cell, tp, k4, &ckey, &cslot, crec, q2);
 #pragma unroll
       for (int c = 0; c < DIM; ++c)
         inc[c] = s6d[c] * ((double)k1[c] + 2.0 * ((double)k2[c] + (double)k3[c]) + (double)k4[c]);

@@ -85,19 +85,24 @@ def test_cell_pair_layout_2d_time_linear_bit_identical():
         assert torch.equal(f1, f0)
 
 
-@pytest.mark.parametrize("unsteady", [False, True])
-def test_cell_layouts_3d_periodic_bit_identical(unsteady):
+@pytest.mark.parametrize("mode", ["steady", "linear", "frozen"])
+def test_cell_layouts_3d_periodic_bit_identical(mode):
+    """The FP32 kernel keeps each lane's last cell record in registers, keyed by record and
+    time slot: steady (one slot), time-linear pairs (slots change with the stage times) and
+    frozen snapshots (a new slot every step) all match the node layouts bit for bit."""
     from pyftle_amd import _native as N
 
     n = 20
-    vels, pts = H.abc_window(n=n, n_snap=5 if unsteady else 1, snap_dt=0.25, unsteady=unsteady)
+    vels, pts = H.abc_window(n=n, n_snap=1 if mode == "steady" else 5, snap_dt=0.25, unsteady=mode != "steady")
     per = (True, True, True)
-    lay0, lay1 = (N.LAYOUT_PAIR, N.LAYOUT_CELL_PAIR) if unsteady else (N.LAYOUT_SINGLE, N.LAYOUT_CELL)
+    lay0, lay1 = (N.LAYOUT_PAIR, N.LAYOUT_CELL_PAIR) if mode == "linear" else (N.LAYOUT_SINGLE, N.LAYOUT_CELL)
     node = _field(vels, pts, (n, n, n), lay0, periodic=per, dt_snap=0.25)
     cell = _field(vels, pts, (n, n, n), lay1, periodic=per, dt_snap=0.25)
     x0 = H.seeds_3d(9, lo=0.05, hi=6.2)  # groups across the periodic seams
-    kw = dict(method="rk4", time_mode="linear" if unsteady else "steady", n_steps=80, h=0.0125,
-              t_start=0.05, map_period=1.0)
+    if mode == "frozen":  # one snapshot per step, five steps
+        kw = dict(method="rk4", time_mode="frozen", n_steps=5, h=0.2, map_period=1.0)
+    else:
+        kw = dict(method="rk4", time_mode=mode, n_steps=80, h=0.0125, t_start=0.05, map_period=1.0)
     f0, p0 = _run(node, x0, **kw)
     f1, p1 = _run(cell, x0, **kw)
     assert torch.equal(p1, p0)
