@@ -49,7 +49,7 @@ extern "C" {
 #define PYFTLE_LAYOUT_SINGLE 0 /* record = one time level: 2D (u,v); 3D (u,v,w,0) */
 #define PYFTLE_LAYOUT_PAIR 1   /* record = levels k,k+1: 2D (u0,v0,u1,v1); 3D (u0,v0,w0,0,u1,v1,w1,0) */
 /* Per-cell records (FP32 GridInterp-mode fields, read by the FP32 advection and batch
- * kernels only): the record of cell (i, j[, k]) holds the 2^dim corner nodes of the cell,
+ * kernels and by pyftle_field_sample): the record of cell (i, j[, k]) holds the 2^dim corner nodes of the cell,
  * corner q = (i + (q&1), j + (q>>1 & 1)[, k + (q>>2 & 1)]), dim floats each, corner after
  * corner; one level (CELL) or levels k, k+1 one after the other (CELL_PAIR).  n cells per
  * axis (the last one holds valid corners on periodic axes only).  One interpolation reads

@@ -331,7 +331,10 @@ __device__ __forceinline__ void sample_field(const FieldDev &f, const char *slot
 // FP32-stored field, any position / blend precision:
 //   Pos   = precision of the position, index and fractional weights;
 //   Blend = precision of the corner blend (lerps with fused multiply-add).
-template <int DIM, bool PAIR, typename Pos, typename Blend>
+// CELL: per-cell records (PYFTLE_LAYOUT_CELL, CELL_PAIR), the cell's corners
+// [level][corner][component] in one record: the same corner values as the node
+// records (upper corner wrapped on periodic axes), so the same result.
+template <int DIM, bool PAIR, typename Pos, typename Blend, bool CELL = false>
 __device__ __forceinline__ void sample_field_f32(const FieldDev &f, const char *slot, Blend w,
                                                  const Pos *x, Blend *vel) {
   constexpr int C = RecInfo<DIM, PAIR>::comps;
@@ -350,6 +353,28 @@ __device__ __forceinline__ void sample_field_f32(const FieldDev &f, const char *
   float c[NQ][C];
   const int nx = f.px;
   const int nxy = f.pxy;
+  if constexpr (CELL) {
+    constexpr int LV = PAIR ? 2 : 1;
+    constexpr int RF = NQ * DIM * LV;  // 8, 16, 24 or 48 floats
+    int cell = ax[1].i0 * nx + ax[0].i0;
+    if constexpr (DIM == 3) cell += ax[2 % DIM].i0 * nxy;
+    const float4 *rp = reinterpret_cast<const float4 *>(slot) + (int64_t)cell * (RF / 4);
+    float r[RF];
+#pragma unroll
+    for (int h = 0; h < RF / 4; ++h) {
+      const float4 v = rp[h];
+      r[4 * h] = v.x;
+      r[4 * h + 1] = v.y;
+      r[4 * h + 2] = v.z;
+      r[4 * h + 3] = v.w;
+    }
+#pragma unroll
+    for (int l = 0; l < LV; ++l)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int comp = 0; comp < DIM; ++comp) c[q][l * L + comp] = r[(l * NQ + q) * DIM + comp];
+  } else {
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     const int ix = (q & 1) ? ax[0].i1 : ax[0].i0;
@@ -362,6 +387,7 @@ __device__ __forceinline__ void sample_field_f32(const FieldDev &f, const char *
 #pragma unroll
       for (int e = 0; e < (C > 4 ? 4 : C); ++e) c[q][h * 4 + e] = r.v[e];
     }
+  }
   }
   const Blend xd = (Blend)ax[0].d, yd = (Blend)ax[1].d;
   const Blend zd = (Blend)ax[2 % DIM].d;

@@ -151,6 +151,21 @@ __global__ void __launch_bounds__(256) sample_kernel(FieldDev f, const char *slo
   for (int a = 0; a < DIM; ++a) out[i * DIM + a] = (P)v[a];
 }
 
+// The same over per-cell records (FP32 fields): bit-identical to the node records.
+template <int DIM, bool PAIR, typename P>
+__global__ void __launch_bounds__(256) sample_cell_kernel(FieldDev f, const char *slot, double w,
+                                                          const P *__restrict__ pts, int64_t n,
+                                                          P *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float x[DIM], v[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) x[a] = (float)pts[i * DIM + a];
+  sample_field_f32<DIM, PAIR, float, float, true>(f, slot, (float)w, x, v);
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) out[i * DIM + a] = (P)v[a];
+}
+
 // RegularGridInterpolator modes (rgi.hpp) and scattered-data modes (scatter.hpp),
 // one lane per point, FP64 arithmetic.
 template <int DIM, int INTERP, typename T, typename P>
@@ -216,6 +231,17 @@ static void launch_sample(const FieldDev &d, const char *slot, double w, const v
         d, slot, w, (const double *)pts, n, (double *)out);
   else
     sample_kernel<DIM, PAIR, Real, float><<<blocks_for(n, 256), 256, 0, st>>>(
+        d, slot, w, (const float *)pts, n, (float *)out);
+}
+
+template <int DIM, bool PAIR>
+static void launch_sample_cell(const FieldDev &d, const char *slot, double w, const void *pts, int pd,
+                               int64_t n, void *out, hipStream_t st) {
+  if (pd == PYFTLE_F64)
+    sample_cell_kernel<DIM, PAIR, double><<<blocks_for(n, 256), 256, 0, st>>>(
+        d, slot, w, (const double *)pts, n, (double *)out);
+  else
+    sample_cell_kernel<DIM, PAIR, float><<<blocks_for(n, 256), 256, 0, st>>>(
         d, slot, w, (const float *)pts, n, (float *)out);
 }
 
@@ -342,8 +368,6 @@ int pyftle_field_sample(const pyftle_field *field, int32_t time_mode, int32_t sn
   if (n == 0) return PYFTLE_OK;
   int k = snapshot;
   double w = 0.0;
-  if (layout_cell(field->layout))
-    return fail("field_sample: the CELL layouts are read by the FP32 advection kernels only");
   const bool pair = layout_pair(field->layout);
   if (time_mode == PYFTLE_TIME_LINEAR) {
     if (!pair) return fail("field_sample: LINEAR time needs the PAIR layout");
@@ -365,6 +389,16 @@ int pyftle_field_sample(const pyftle_field *field, int32_t time_mode, int32_t sn
   const char *slot = reinterpret_cast<const char *>(field->data) + (int64_t)s * field->slot_stride;
   hipStream_t st = as_stream(stream);
   const bool f64 = field->dtype == PYFTLE_F64;
+  if (layout_cell(field->layout)) {  // FP32 GridInterp fields only (checked when created)
+    if (field->dim == 2) {
+      if (pair) launch_sample_cell<2, true>(d, slot, w, points, pts_dtype, n, out, st);
+      else launch_sample_cell<2, false>(d, slot, w, points, pts_dtype, n, out, st);
+    } else {
+      if (pair) launch_sample_cell<3, true>(d, slot, w, points, pts_dtype, n, out, st);
+      else launch_sample_cell<3, false>(d, slot, w, points, pts_dtype, n, out, st);
+    }
+    return check_launch("field_sample");
+  }
   if (field->interp != PYFTLE_INTERP_GINTERP) {
     if (field->dim == 2) launch_sample_rgi<2>(d, f64, slot, points, pts_dtype, n, out, st);
     else launch_sample_rgi<3>(d, f64, slot, points, pts_dtype, n, out, st);

@@ -4,7 +4,8 @@ A cell record holds the same FP32 values as the 2^dim node records of the cell a
 kernel does the same arithmetic on them, so final positions and FTLE are bit-identical:
 periodic and bounded axes, particles that leave the domain (zero velocity outside), every
 integrator, steady / frozen / time-linear snapshots, single launches and batches.  The
-layouts are read by the FP32 advection kernels only; other uses are refused.
+layouts hold FP32 GridInterp fields, read by the FP32 advection kernels and the gather call;
+other uses are refused.
 """
 
 import numpy as np
@@ -172,5 +173,32 @@ def test_cell_layouts_refuse_other_uses():
                       cfg=W.LaunchConfig(compute=N.COMPUTE_MIXED))
     with pytest.raises(RuntimeError, match="LINEAR time needs"):
         W.advect_ftle(f, x0, method="rk4", time_mode="linear", n_steps=2, h=0.1, map_period=0.2)
-    with pytest.raises(RuntimeError, match="FP32 advection kernels only"):
-        f.sample(torch.zeros(4, 2, device="cuda", dtype=torch.float64))
+
+
+@pytest.mark.parametrize("pts_dtype", [torch.float64, torch.float32])
+def test_cell_layouts_field_sample_bit_identical(pts_dtype):
+    """The gather call reads the cell records too, with the node layouts' result bit for bit:
+    2D time-linear pairs on a bounded grid (points inside, on the edges and outside) and a
+    3D periodic field (points beyond the seams wrap)."""
+    from pyftle_amd import _native as N
+
+    vels, pts = H.double_gyre_window(nx=33, ny=17, n_snap=4)
+    node = _field(vels, pts, (33, 17), N.LAYOUT_PAIR)
+    cell = _field(vels, pts, (33, 17), N.LAYOUT_CELL_PAIR)
+    rng = np.random.default_rng(3)
+    q = np.concatenate([rng.uniform((-0.2, -0.1), (2.2, 1.1), size=(500, 2)),
+                        [[0.0, 0.0], [2.0, 1.0], [2.0, 0.5], [1.0, 1.0]]])
+    q = torch.as_tensor(q, device="cuda", dtype=pts_dtype)
+    for t in (0.0, 0.13, 0.29):
+        a = node.sample(q, time_mode=N.TIME_LINEAR, t=t)
+        b = cell.sample(q, time_mode=N.TIME_LINEAR, t=t)
+        assert torch.equal(a, b)
+    assert (a != 0).any() and (a == 0).any()  # inside and outside points
+
+    n = 12
+    vels, pts = H.abc_window(n=n, n_snap=1)
+    per = (True, True, True)
+    node = _field(vels, pts, (n, n, n), N.LAYOUT_SINGLE, periodic=per)
+    cell = _field(vels, pts, (n, n, n), N.LAYOUT_CELL, periodic=per)
+    q = torch.as_tensor(rng.uniform(-1.0, 7.5, size=(700, 3)), device="cuda", dtype=pts_dtype)
+    assert torch.equal(node.sample(q, time_mode=N.TIME_STEADY), cell.sample(q, time_mode=N.TIME_STEADY))
