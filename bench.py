@@ -399,7 +399,7 @@ def build_surrogate3d(torch, n=192, ns=256, t0s=(0.0, 2.5, 5.0, 7.5), members=4,
                         "t0_values": list(t0s), "T": 5.0, "rk4_h": 0.01,
                         "mlp": "64 Fourier freqs (sigma 0.5) on (x,y,z,t0), 6x512 SiLU -> 3",
                         "members": members, "epochs": epochs, "trajectories": n_traj,
-                        "judge": "RK4 on a 1% held-out seed subset (rng seed 1)",
+                        "accuracy_reference": "RK4 on a 1% held-out seed subset (rng seed 1)",
                         "dtype": "bf16 MFMA, fp32 accumulate"})
 
 
@@ -430,8 +430,8 @@ def run_surrogate3d(torch, wl, args):
     seeds = torch.stack([cx.reshape(-1), cy.reshape(-1), cz.reshape(-1)], 1).contiguous()
     del cx, cy, cz
     g = torch.Generator(device=dev).manual_seed(1)
-    judge_idx = torch.randperm(seeds.shape[0], generator=g, device=dev)[: seeds.shape[0] // 100].sort().values
-    judge_seeds = seeds[judge_idx].contiguous()
+    check_idx = torch.randperm(seeds.shape[0], generator=g, device=dev)[: seeds.shape[0] // 100].sort().values
+    check_seeds = seeds[check_idx].contiguous()
     sf = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], tol=args.surrogate_tol, chunk=1 << 19)
 
     def timed(fn, reps=1):
@@ -466,15 +466,15 @@ def run_surrogate3d(torch, wl, args):
         f"({rk4_node_ms:.1f} ms node records; max FTLE difference / range {layout_diff:.1e})")
     per_t0 = []
     for t0 in wl["t0s"]:
-        ref = sf.rk4_ftle(judge_seeds, t0)
+        ref = sf.rk4_ftle(check_seeds, t0)
         row = {"t0": t0}
         for jm in (wl["members"], 1):
             sf.jacobian_members = jm
             (f_all, spread), ms = timed(lambda: sf.predict(seeds, t0))
-            f_j = f_all[judge_idx]
+            f_j = f_all[check_idx]
             row[f"jac{jm}"] = {"ms": ms, "rmse": SF.relative_rmse(f_j, ref)}
             if jm == wl["members"]:
-                sp_j = spread[judge_idx]
+                sp_j = spread[check_idx]
                 for tol in (1e-2, 3e-2, 1e-1, 3e-1):
                     bad = sp_j > tol
                     comp = torch.where(bad, ref, f_j)

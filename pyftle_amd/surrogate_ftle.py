@@ -1,4 +1,4 @@
-"""FTLE fields from the learned flow-map surrogate, with RK4 as fallback and judge.
+"""FTLE fields from the learned flow-map surrogate, with RK4 as fallback and accuracy reference.
 
 Pipeline:
 

@@ -1,4 +1,4 @@
-"""GPU: the learned flow-map surrogate against plain RK4 (its fallback and judge).
+"""GPU: the learned flow-map surrogate against plain RK4 (its fallback and accuracy reference).
 
 * trained on RK4 trajectories of a small double gyre, the ensemble's FTLE is
   within 10% (RMSE / range) of RK4;
