@@ -18,6 +18,12 @@ Pins (checked in ``tests/test_oracle.py``):
   ``numpy.linalg.eigvals`` and closed forms (identity -> 0, lambda_max = 4).
 * window driver: closed-form saddle ``u = l x, v = -l y`` and the window
   arithmetic of ``ftle_solver.py:99-160``.
+* time-linear driver (``time_linear_velocity``, ``run_window_linear``): closed
+  forms on time-dependent snapshots -- a uniform ``u(t) = a + b t`` (positions
+  exact for RK4, Euler and AB2; the blend exact between snapshots) and a stretch
+  ``u = c(t) x`` with ``c`` linear in ``t`` (FTLE = the mean of ``c`` over the
+  window, divided by ``n |h|``); mutated drivers (``k3`` at ``t + h``, every
+  stage at ``t``, a divisor of ``(n - 1)|h|``) fail these checks.
 * ``RGIField``: the reference's ``GridInterpolator`` calls SciPy's
   ``RegularGridInterpolator`` (SciPy 1.15.3 here, the version the reference
   pins); the oracle makes the same calls.  Pinned by closed forms: multilinear

@@ -97,6 +97,50 @@ def test_window_time_linear_2d(precision):
         assert err <= 1e-3
 
 
+@pytest.mark.parametrize("method", ["rk4", "euler", "ab2"])
+def test_window_time_linear_closed_forms(method):
+    """The time-linear kernel against closed forms, not the oracle: snapshots of the
+    uniform u(t) = a + b t (positions exact for RK4 as Simpson's rule, and for the
+    Euler/AB2 sums) and of the stretch u = c(t) x, v = -c(t) y (RK4 FTLE = mean of c)."""
+    from oracle import flows
+    from pyftle_amd import window as W
+
+    a, b = np.array([0.3, -0.2]), np.array([0.1, 0.05])
+    t0, h, n = 0.35, 0.05, 40
+    T = n * h
+    x, y = np.linspace(-10.0, 10.0, 9), np.linspace(-10.0, 10.0, 7)
+    vels = []
+    for k in range(12):
+        v, pts = flows.sample_2d(lambda t, q: np.broadcast_to(a + b * t, q.shape).astype(np.float64),
+                                 0.25 * k, x, y)
+        vels.append(v)
+    f = _field_2d(vels, pts, (9, 7), "float64", pair=True, dt_snap=0.25)
+    x0 = O.make_neighbor_positions(np.array([[0.4, -0.3], [1.1, 0.7], [-2.0, 1.5]]), 1e-3)
+    pos = torch.as_tensor(x0, device="cuda")
+    out = torch.empty_like(pos)
+    W.advect_ftle(f, pos, method=method, time_mode="linear", n_steps=n, h=h, t_start=t0,
+                  map_period=T, positions_out=out)
+    ti = t0 + h * np.arange(n)
+    if method == "rk4":
+        exp = x0 + a * T + b * (t0 * T + 0.5 * T * T)
+    elif method == "euler":
+        exp = x0 + h * (n * a + b * ti.sum())
+    else:
+        exp = x0 + h * (a + b * t0) + h * ((n - 1) * a + b * (ti[1:].sum() + 0.5 * h * (n - 1)))
+    np.testing.assert_allclose(out.cpu().numpy(), exp, rtol=0, atol=1e-12)
+    if method != "rk4":
+        return
+    c0, c1 = 0.4, 0.2
+    g = np.linspace(-3.0, 3.0, 13)
+    vels = [flows.sample_2d(lambda t, q: flows.saddle(t, q, c0 + c1 * 0.5 * k), 0.0, g, g)[0] for k in range(7)]
+    pts = flows.sample_2d(flows.saddle, 0.0, g, g)[1]
+    f = _field_2d(vels, pts, (13, 13), "float64", pair=True, dt_snap=0.5)
+    x0 = O.make_neighbor_positions(np.array([[0.05, 0.2], [-0.1, 0.3], [0.2, -0.25]]), 1e-3)
+    ftle = W.advect_ftle(f, torch.as_tensor(x0, device="cuda"), method="rk4", time_mode="linear",
+                         n_steps=n, h=h, t_start=0.3, map_period=n * h)
+    np.testing.assert_allclose(ftle.cpu().numpy(), c0 + c1 * (0.3 + 0.5 * n * h), rtol=1e-7)
+
+
 @pytest.mark.parametrize("precision", ["float64", "float32"])
 def test_window_3d_periodic_abc(precision):
     from pyftle_amd import _native as N

@@ -251,3 +251,118 @@ def test_rgi_field_oracle_closed_forms():
     q3 = np.random.default_rng(1).uniform([-1, 0, 0.5], [2, 1, 1.5], (100, 3))
     v3 = O.RGIField((nx, ny, nz), vel3, pts3, "linear").interpolate(q3)
     np.testing.assert_allclose(v3[:, 2], f3(q3[:, 0], q3[:, 1], q3[:, 2]), atol=1e-13)
+
+
+# ------------------------------------------------- time-linear window driver --
+# run_window_linear / time_linear_velocity have no reference counterpart (the
+# reference freezes one snapshot per step, ftle_solver.py:99-102).  They are pinned
+# here by closed forms on time-dependent snapshots; each check is shown to reject
+# the plausible slips of such a driver (a stage at the wrong time, a wrong divisor,
+# an off-by-one snapshot index) by running mutated drivers through it.
+
+def _uniform_field_snapshots(a, b, snap_t0, snap_dt, n_snap):
+    """Snapshot k of u(t) = a + b t (spatially uniform) at t_k = snap_t0 + k snap_dt."""
+    x, y = np.linspace(-10.0, 10.0, 9), np.linspace(-10.0, 10.0, 7)
+    out = []
+    for k in range(n_snap):
+        def fn(t, q):
+            return np.broadcast_to(a + b * t, q.shape).astype(np.float64)
+
+        vel, pts = flows.sample_2d(fn, snap_t0 + k * snap_dt, x, y)
+        out.append(O.GridField((9, 7), vel, pts).interpolate)
+    return out
+
+
+def _stretch_field_snapshots(c0, c1, snap_t0, snap_dt, n_snap):
+    """Snapshot k of u = c(t) x, v = -c(t) y with c(t) = c0 + c1 t (exact under bilinear)."""
+    x = np.linspace(-3.0, 3.0, 13)
+    out = []
+    for k in range(n_snap):
+        c = c0 + c1 * (snap_t0 + k * snap_dt)
+        vel, pts = flows.sample_2d(lambda t, q: flows.saddle(t, q, c), 0.0, x, x)
+        out.append(O.GridField((13, 13), vel, pts).interpolate)
+    return out
+
+
+def _mutant_window_linear(velocity_at, positions, t0, h, n_steps, kind):
+    """run_window_linear with one planted slip (test-only; never used as the oracle)."""
+    x = np.array(positions, dtype=np.float64, copy=True)
+    x0 = x.copy()
+    for i in range(n_steps):
+        t = t0 + i * h
+        if kind == "k3_at_t_plus_h":
+            f1, f2, f3, f4 = velocity_at(t), velocity_at(t + 0.5 * h), velocity_at(t + h), velocity_at(t + h)
+        elif kind == "all_stages_at_t":
+            f1 = f2 = f3 = f4 = velocity_at(t)
+        else:
+            f1, f2, f3, f4 = velocity_at(t), velocity_at(t + 0.5 * h), velocity_at(t + 0.5 * h), velocity_at(t + h)
+        k1 = f1(x)
+        k2 = f2(x + 0.5 * h * k1)
+        k3 = f3(x + 0.5 * h * k2)
+        k4 = f4(x + h * k3)
+        x += (h / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
+    T = (n_steps - 1) * abs(h) if kind == "divisor_n_minus_1" else n_steps * abs(h)
+    return O.ftle_from_positions(x, x0, T), x
+
+
+_A = np.array([0.3, -0.2])
+_B = np.array([0.1, 0.05])
+
+
+def test_time_linear_velocity_closed_form_between_and_beyond_snapshots():
+    """Snapshots of u(t) = a + b t: the blend is exact at any t inside the series
+    (snapshot index floor((t - t0)/dt), weight the remainder) and clamps outside."""
+    snaps = _uniform_field_snapshots(_A, _B, 0.5, 0.25, 9)   # t in [0.5, 2.5]
+    vat = O.time_linear_velocity(snaps, 0.5, 0.25)
+    q = np.array([[0.1, 0.2], [-1.0, 3.0]])
+    for t in (0.5, 0.61, 0.75, 1.3, 2.2499, 2.5):
+        np.testing.assert_allclose(vat(t)(q), np.broadcast_to(_A + _B * t, q.shape), rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(vat(0.1)(q), np.broadcast_to(_A + _B * 0.5, q.shape), rtol=1e-14)
+    np.testing.assert_allclose(vat(9.0)(q), np.broadcast_to(_A + _B * 2.5, q.shape), rtol=1e-14)
+    # an off-by-one snapshot index or a swapped weight would break the interior values
+    shifted = O.time_linear_velocity(snaps, 0.5 - 0.25, 0.25)
+    assert not np.allclose(shifted(1.3)(q), _A + _B * 1.3, rtol=1e-6)
+
+
+def test_window_linear_uniform_field_closed_form_and_mutants():
+    """u(t) = a + b t is quadratic-exact for RK4 (Simpson): x(T) = x0 + a T + b (t0 T + T^2/2)
+    with stages at t, t+h/2, t+h/2, t+h; k3 at t+h or every stage at t break it."""
+    t0, h, n = 0.35, 0.05, 40
+    T = n * h
+    snaps = _uniform_field_snapshots(_A, _B, 0.0, 0.25, 12)   # t in [0, 2.75] covers [0.35, 2.35]
+    vat = O.time_linear_velocity(snaps, 0.0, 0.25)
+    x0 = O.make_neighbor_positions(np.array([[0.4, -0.3], [1.1, 0.7], [-2.0, 1.5]]), 1e-3)
+    exp = x0 + _A * T + _B * (t0 * T + 0.5 * T * T)
+    _, x = O.run_window_linear(vat, x0, t0, h, n, "rk4")
+    np.testing.assert_allclose(x, exp, rtol=0, atol=1e-12)
+    for kind in ("k3_at_t_plus_h", "all_stages_at_t"):
+        _, xm = _mutant_window_linear(vat, x0, t0, h, n, kind)
+        assert np.max(np.abs(xm - exp)) > 1e-5, kind
+    # Euler: x_n = x0 + h sum_i (a + b t_i); AB2: Euler first, then h(a + b t_i + b h / 2)
+    ti = t0 + h * np.arange(n)
+    _, xe = O.run_window_linear(vat, x0, t0, h, n, "euler")
+    np.testing.assert_allclose(xe, x0 + h * (n * _A + _B * ti.sum()), rtol=0, atol=1e-12)
+    _, xa = O.run_window_linear(vat, x0, t0, h, n, "ab2")
+    exp_ab2 = x0 + h * (_A + _B * t0) + h * ((n - 1) * _A + _B * (ti[1:].sum() + 0.5 * h * (n - 1)))
+    np.testing.assert_allclose(xa, exp_ab2, rtol=0, atol=1e-12)
+
+
+def test_window_linear_stretch_ftle_closed_form_and_mutants():
+    """u = c(t) x, v = -c(t) y, c = c0 + c1 t: the x separation grows by exp(int c dt), so
+    FTLE = (1/(n|h|)) int_{t0}^{t0+nh} c dt = c0 + c1 (t0 + n h / 2) (RK4 error ~1e-9 here);
+    a divisor of (n-1)|h| or a stage-time slip is off by more than 1e-4 relative."""
+    c0, c1, t0, h, n = 0.4, 0.2, 0.3, 0.05, 40
+    snaps = _stretch_field_snapshots(c0, c1, 0.0, 0.5, 7)      # t in [0, 3] covers [0.3, 2.3]
+    vat = O.time_linear_velocity(snaps, 0.0, 0.5)
+    x0 = O.make_neighbor_positions(np.array([[0.05, 0.2], [-0.1, 0.3], [0.2, -0.25]]), 1e-3)
+    exact = c0 + c1 * (t0 + 0.5 * n * h)
+    ftle, x = O.run_window_linear(vat, x0, t0, h, n, "rk4")
+    np.testing.assert_allclose(ftle, exact, rtol=1e-7)
+    growth = np.exp(c0 * n * h + c1 * ((t0 + n * h) ** 2 - t0 ** 2) / 2)
+    np.testing.assert_allclose(x[:, 0], x0[:, 0] * growth, rtol=1e-7)
+    for kind in ("k3_at_t_plus_h", "all_stages_at_t", "divisor_n_minus_1"):
+        fm, _ = _mutant_window_linear(vat, x0, t0, h, n, kind)
+        assert np.max(np.abs(fm / exact - 1)) > 1e-4, kind
+    # the unmutated restatement inside the helper agrees with the oracle
+    fr, _ = _mutant_window_linear(vat, x0, t0, h, n, "none")
+    np.testing.assert_allclose(fr, ftle, rtol=1e-12)
