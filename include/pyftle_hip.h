@@ -306,6 +306,26 @@ int pyftle_batch_info(const pyftle_batch *batch, int64_t *n_items, int32_t *grid
                       int32_t *block_threads);
 int pyftle_batch_destroy(pyftle_batch *batch);
 
+/* ---- batched windows without device allocation ----
+ * No reference counterpart (pyFTLE allocates nothing on a GPU).  For callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc -- this runs one batch like
+ * pyftle_batch_create + pyftle_batch_launch (same validation, same per-item
+ * arithmetic, bit-identical FTLE and positions) without allocating: the
+ * descriptor table, the prefix sums of the fields' block counts and the work
+ * counter live in one static device array of PYFTLE_BATCH_NOALLOC_SCRATCH_BYTES
+ * per GPU, which the library keeps until the process ends.  At most
+ * PYFTLE_BATCH_NOALLOC_MAX_FIELDS fields per call (more -> PYFTLE_EINVAL).
+ * opts->use_graph is ignored (nothing is captured).  Enqueued on `stream`; the
+ * descriptors are copied from host memory before the call returns, so `fields`
+ * may be reused at once.  Every call on one GPU shares that GPU's array (the
+ * counter is cleared by a kernel at each call): calls that overlap on one GPU
+ * (other host threads, or other streams whose work has not finished) race. */
+#define PYFTLE_BATCH_NOALLOC_MAX_FIELDS 256
+#define PYFTLE_BATCH_NOALLOC_SCRATCH_BYTES (192 * 1024)
+int pyftle_batch_run_noalloc(const pyftle_batch_field *fields, int32_t n_fields,
+                             const pyftle_batch_opts *opts, void *stream);
+
 /* ---- flow-map surrogate: Fourier-feature MLP kernels ----
  * No reference counterpart: a learned approximation of the flow map that
  * pyFTLE's RK4 path (integrate.py:265-342) computes; RK4 stays the fallback

@@ -61,7 +61,11 @@ _SIGS = {
     "pyftle_batch_info": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
                           ctypes.POINTER(ctypes.c_int32)],
     "pyftle_batch_destroy": [ctypes.c_void_p],
+    "pyftle_batch_run_noalloc": [ctypes.POINTER(BatchField), ctypes.c_int32, ctypes.POINTER(BatchOpts),
+                                 ctypes.c_void_p],
 }
+
+NOALLOC_MAX_FIELDS = 256  # PYFTLE_BATCH_NOALLOC_MAX_FIELDS
 
 
 @dataclass
@@ -89,6 +93,58 @@ class Window:
     order: torch.Tensor | None = None  # int32 device permutation of the seed groups (window.seed_order)
 
 
+def _describe(windows: list[Window], method, time_mode, cfg: BatchConfig):
+    """Checks the windows and fills the ``pyftle_batch_field`` array and options;
+    allocates each window's missing FTLE output."""
+    if not windows:
+        raise ValueError("a batch needs at least one window")
+    pdt = windows[0].positions.dtype
+    arr = (BatchField * len(windows))()
+    for i, w in enumerate(windows):
+        dim = w.field.geom.dim
+        p = w.positions
+        if p.dtype != pdt or not p.is_cuda or not p.is_contiguous():
+            raise ValueError("batch positions must be contiguous device tensors of one dtype")
+        if p.dim() != 2 or p.shape[1] != dim or p.shape[0] % (2 * dim) != 0:
+            raise ValueError(f"window {i}: positions must have shape (2*{dim}*N, {dim}), got {tuple(p.shape)}")
+        n = p.shape[0] // (2 * dim)
+        po = w.positions_out
+        if po is not None and (po.shape != p.shape or po.dtype != p.dtype or not po.is_cuda
+                               or not po.is_contiguous()):
+            raise ValueError(f"window {i}: positions_out must be a contiguous device tensor like positions")
+        if w.ftle is None:
+            w.ftle = torch.empty(n, dtype=torch.float64, device=p.device)
+        elif (w.ftle.dtype != torch.float64 or w.ftle.numel() != n or not w.ftle.is_cuda
+              or not w.ftle.is_contiguous()):
+            raise ValueError(f"window {i}: ftle must be a contiguous float64 device tensor of {n} values")
+        bf = arr[i]
+        bf.field = w.field.desc
+        bf.positions_in = w.positions.data_ptr()
+        bf.positions_out = w.positions_out.data_ptr() if w.positions_out is not None else None
+        bf.ftle_out = w.ftle.data_ptr()
+        bf.n_groups = n
+        bf.n_steps = int(w.n_steps)
+        bf.first_snapshot = int(w.first_snapshot)
+        bf.t_start = float(w.t_start)
+        bf.h = float(w.h)
+        bf.map_period = float(w.map_period)
+        if w.order is not None:
+            if w.order.dtype != torch.int32 or not w.order.is_cuda or w.order.numel() != n:
+                raise ValueError(f"window {i}: order must be an int32 device permutation of {n} groups")
+            w.order = w.order.contiguous()
+            bf.seed_order = w.order.data_ptr()
+    o = BatchOpts()
+    o.method = METHODS[method] if isinstance(method, str) else int(method)
+    o.time_mode = TIME_MODES[time_mode] if isinstance(time_mode, str) else int(time_mode)
+    o.pos_dtype = dtype_code(pdt)
+    o.groups_per_block = cfg.groups_per_block
+    o.lane_order = cfg.lane_order
+    o.blocks_per_cu = cfg.blocks_per_cu
+    o.longest_first = cfg.longest_first
+    o.use_graph = cfg.use_graph
+    return arr, o
+
+
 class BatchPlan:
     """A captured batch: ``run()`` recomputes every window's FTLE in one launch.
 
@@ -105,50 +161,7 @@ class BatchPlan:
         self.cfg = cfg or BatchConfig()
         if not windows:
             raise ValueError("a batch needs at least one window")
-        pdt = windows[0].positions.dtype
-        arr = (BatchField * len(windows))()
-        for i, w in enumerate(windows):
-            dim = w.field.geom.dim
-            p = w.positions
-            if p.dtype != pdt or not p.is_cuda or not p.is_contiguous():
-                raise ValueError("batch positions must be contiguous device tensors of one dtype")
-            if p.dim() != 2 or p.shape[1] != dim or p.shape[0] % (2 * dim) != 0:
-                raise ValueError(f"window {i}: positions must have shape (2*{dim}*N, {dim}), got {tuple(p.shape)}")
-            n = p.shape[0] // (2 * dim)
-            po = w.positions_out
-            if po is not None and (po.shape != p.shape or po.dtype != p.dtype or not po.is_cuda
-                                   or not po.is_contiguous()):
-                raise ValueError(f"window {i}: positions_out must be a contiguous device tensor like positions")
-            if w.ftle is None:
-                w.ftle = torch.empty(n, dtype=torch.float64, device=p.device)
-            elif (w.ftle.dtype != torch.float64 or w.ftle.numel() != n or not w.ftle.is_cuda
-                  or not w.ftle.is_contiguous()):
-                raise ValueError(f"window {i}: ftle must be a contiguous float64 device tensor of {n} values")
-            bf = arr[i]
-            bf.field = w.field.desc
-            bf.positions_in = w.positions.data_ptr()
-            bf.positions_out = w.positions_out.data_ptr() if w.positions_out is not None else None
-            bf.ftle_out = w.ftle.data_ptr()
-            bf.n_groups = n
-            bf.n_steps = int(w.n_steps)
-            bf.first_snapshot = int(w.first_snapshot)
-            bf.t_start = float(w.t_start)
-            bf.h = float(w.h)
-            bf.map_period = float(w.map_period)
-            if w.order is not None:
-                if w.order.dtype != torch.int32 or not w.order.is_cuda or w.order.numel() != n:
-                    raise ValueError(f"window {i}: order must be an int32 device permutation of {n} groups")
-                w.order = w.order.contiguous()
-                bf.seed_order = w.order.data_ptr()
-        o = BatchOpts()
-        o.method = METHODS[method] if isinstance(method, str) else int(method)
-        o.time_mode = TIME_MODES[time_mode] if isinstance(time_mode, str) else int(time_mode)
-        o.pos_dtype = dtype_code(pdt)
-        o.groups_per_block = self.cfg.groups_per_block
-        o.lane_order = self.cfg.lane_order
-        o.blocks_per_cu = self.cfg.blocks_per_cu
-        o.longest_first = self.cfg.longest_first
-        o.use_graph = self.cfg.use_graph
+        arr, o = _describe(windows, method, time_mode, self.cfg)
         h = ctypes.c_void_p()
         N.call("pyftle_batch_create", arr, len(windows), ctypes.byref(o), ctypes.byref(h))
         self._h = h
@@ -175,6 +188,21 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+def run_batch_noalloc(windows: list[Window], method="rk4", time_mode="linear",
+                      cfg: BatchConfig | None = None, stream=None) -> list[torch.Tensor]:
+    """One batch through ``pyftle_batch_run_noalloc``: the same FTLE fields as
+    ``BatchPlan(windows, ...).run()`` with no device allocation by the library (its
+    descriptors and work counter live in a static per-GPU array).  Each window's
+    ``ftle`` must already be allocated, or is allocated here by torch before the
+    call.  Calls that overlap on one GPU race on that array."""
+    N.extra_signatures(_SIGS)
+    if len(windows) > NOALLOC_MAX_FIELDS:
+        raise ValueError(f"at most {NOALLOC_MAX_FIELDS} windows per allocation-free batch")
+    arr, o = _describe(windows, method, time_mode, cfg or BatchConfig())
+    N.call("pyftle_batch_run_noalloc", arr, len(windows), ctypes.byref(o), stream_handle(stream))
+    return [w.ftle for w in windows]
 
 
 def autotune_batch(windows, method="rk4", time_mode="linear", repeats=2):
