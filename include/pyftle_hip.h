@@ -220,6 +220,24 @@ int pyftle_field_sample(const pyftle_field *field, int32_t time_mode, int32_t sn
 int pyftle_advect(const pyftle_field *field, const pyftle_advect_opts *opts, void *positions,
                   int32_t pos_dtype, int64_t n_particles, void *vprev, void *stream);
 
+/* ---- integrator stage arithmetic for caller-evaluated velocities: integrate.py:16-109 ----
+ * The analytical strategy evaluates the user's velocity_fn(t, points) on device
+ * arrays at every stage (interpolate.py:765-812); these element-wise updates,
+ * in the reference's operation order, replace euler_step_inplace,
+ * adams_bashforth_2_step_inplace and runge_kutta_4_step_inplace plus the stage
+ * positions of RungeKutta4Integrator.integrate (integrate.py:291-342).
+ * x, k1..k4, out: n contiguous elements of `dtype` (n = particles * dim).
+ *   STAGE: out = x + c*k1          c = 0.5*h or h  (tmp of integrate.py:327-338)
+ *   RK4:   x = x + c*(k1 + 2 k2 + 2 k3 + k4)       c = h/6 (integrate.py:109)
+ *   EULER: x = x + c*k1; out (may be NULL) = k1    c = h   (integrate.py:44)
+ *   AB2:   x = x + c*(1.5 k1 - 0.5 k2); out (may be NULL, may equal k2) = k1 */
+#define PYFTLE_COMBINE_STAGE 0
+#define PYFTLE_COMBINE_RK4 1
+#define PYFTLE_COMBINE_EULER 2
+#define PYFTLE_COMBINE_AB2 3
+int pyftle_stage_combine(int32_t op, int32_t dtype, int64_t n, double c, void *x, const void *k1,
+                         const void *k2, const void *k3, const void *k4, void *out, void *stream);
+
 /* ---- seed order (no reference counterpart): Morton (Z-order) code of the cell
  * floor((centroid - lo) * inv_cell) (clamped to [0, 2^20) per axis) of each seed group of a
  * block-ordered (nb*n_groups, dim) position array; codes (n_groups,) in int64 storage.

@@ -6,7 +6,9 @@
   Arnold-Beltrami-Childress field, ``u = A sin z + C cos y``,
   ``v = B sin x + A cos z``, ``w = C sin y + B cos x``;
 * saddle ``u = l x, v = -l y``: linear field, so bilinear interpolation and
-  the flow map are exact (used for closed-form end-to-end checks).
+  the flow map are exact (used for closed-form end-to-end checks);
+* helical vortex filament: examples/01_Closed-form_velocity_helical.ipynb
+  (cell 1), normalised by the largest speed over the queried points.
 
 All functions take ``points`` of shape ``(M, dim)`` and return ``(M, dim)``.
 Grid samplers return the reference's ``(dim, Ngrid)`` velocity and coordinate
@@ -46,6 +48,29 @@ def saddle(t, points, lam=0.5):
     out[:, 1] = -lam * points[:, 1]
     if points.shape[1] == 3:
         out[:, 2] = 0.0 * points[:, 2]
+    return out
+
+
+def helical_vortex(t, points, Lz=8.0, R0=1.0, Gamma=5.0, sigma=0.3, omega=2.0 * np.pi / 2.0):
+    """examples/01_Closed-form_velocity_helical.ipynb cell 1: swirl around a helical
+    centreline rotating with phase omega*t, a weak axial part, divided by the
+    largest speed among ``points`` (a reduction over the whole query)."""
+    x, y, z = points[:, 0], points[:, 1], points[:, 2]
+    k = 8.0 * np.pi / Lz
+    phi = omega * t
+    xr = x - R0 * np.cos(k * z + phi)
+    yr = y - R0 * np.sin(k * z + phi)
+    r = np.sqrt(xr**2 + yr**2)
+    theta = np.arctan2(yr, xr)
+    v_theta = Gamma / (2.0 * np.pi * r) * (1.0 - np.exp(-((r / sigma) ** 2)))
+    u = -v_theta * np.sin(theta)
+    v = v_theta * np.cos(theta)
+    w = 0.2 * Gamma / (2.0 * np.pi * R0) * np.exp(-((r / sigma) ** 2))
+    scale = np.max(np.sqrt(u**2 + v**2 + w**2))
+    out = np.empty_like(points, dtype=np.float64)
+    out[:, 0] = u / scale
+    out[:, 1] = v / scale
+    out[:, 2] = w / scale
     return out
 
 

@@ -22,6 +22,7 @@ COMPUTE_NATIVE, COMPUTE_MIXED, COMPUTE_F64 = 0, 1, 2
 INTERP_GINTERP, INTERP_RGI_LINEAR, INTERP_RGI_NEAREST, INTERP_RGI_CUBIC = 0, 1, 2, 3
 INTERP_SCATTER_LINEAR, INTERP_SCATTER_NEAREST, INTERP_SCATTER_CUBIC = 4, 5, 6
 OK, EINVAL, EHIP = 0, -1, -2
+COMBINE_STAGE, COMBINE_RK4, COMBINE_EULER, COMBINE_AB2 = 0, 1, 2, 3
 
 
 class Field(ctypes.Structure):
@@ -111,6 +112,7 @@ SIGNATURES = {
     "pyftle_pack_spline": [_FP, _i32, _vp, _i32, ctypes.POINTER(ctypes.c_int64), _vp, _vp],
     "pyftle_field_sample": [_FP, _i32, _i32, _f64, _vp, _i32, _i64, _vp, _vp],
     "pyftle_advect": [_FP, _OP, _vp, _i32, _i64, _vp, _vp],
+    "pyftle_stage_combine": [_i32, _i32, _i64, _f64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pyftle_seed_morton_codes": [_vp, _i32, _i64, _i32, _i32, ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(ctypes.c_double), _vp, _vp],
     "pyftle_advect_ftle": [_FP, _OP, _vp, _i32, _i64, _vp, _f64, _vp, _vp],

@@ -7,10 +7,13 @@ shared by every window (api.py:112,177-183), results stacked to
 
 Windows run one after another on the GPU instead of in a process pool; each
 starts from the caller's original particle positions, as every pickled task
-of the reference's pool does.  The analytical velocity is evaluated at the
-reference's frozen time 0 (interpolate.py:788,812) on a device grid; pass
-``grid=(lower, upper, shape)`` to choose that grid (default: the particles'
-bounding box padded by 25% per side, 1024^2 in 2D / 128^3 in 3D).
+of the reference's pool does.  The analytical velocity is evaluated as in the
+reference, at its frozen time 0 (interpolate.py:788,812) at every integrator
+stage, on device tensors: ``velocity_fn(t, points)`` receives a CUDA tensor
+and returns one (torch operations).  ``grid=(lower, upper, shape)`` opts in to
+sampling the function once onto a device grid of that box and resolution
+(``precision`` sets its storage), read by the fused window kernel; a particle
+leaving that box raises ``ValueError``.
 """
 
 from __future__ import annotations

@@ -15,9 +15,13 @@ keeps the reference signature and result.  ``run()``:
 4. progress messages ``(id, i)`` and ``(id, "done")`` go to the queue
    (ftle_solver.py:104-110); the field is written or returned (112-120).
 
-With an ``AnalyticalInterpolator`` the velocity is evaluated at the
-reference's frozen time 0 (interpolate.py:788,812) on a device grid and the
-window runs in steady mode.
+With an ``AnalyticalInterpolator`` the window steps as the reference does
+(ftle_solver.py:99-102 with interpolate.py:788,812): ``num_steps`` steps, each
+calling the user's ``velocity_fn(0.0, points)`` on the device stage positions
+at every stage, the stages combined by HIP kernels; then the FP64 FTLE kernel.
+With an opt-in grid (``set_grid``) the field sampled once at time 0 is read by
+the fused window kernel in steady mode instead, and the window raises if a
+particle left the sampled box.
 """
 
 from __future__ import annotations
@@ -59,12 +63,20 @@ class FTLESolver:
         map_period = (num_steps - 1) * abs(timestep)
         method = self._method()
 
-        if isinstance(interp, AnalyticalInterpolator):
-            grid = interp.gridded(pos)
-            field, k = grid.device_field()
+        if isinstance(interp, AnalyticalInterpolator) and not interp.is_gridded:
+            dev = dev.contiguous()
+            for i in range(num_steps):
+                self.source.update_interpolator(interp, i)
+                self.integrator.step_device(timestep, dev)
+                if self.progress_queue:
+                    self.progress_queue.put((id_, i))
+            ftle = W.ftle_from_groups(dev, init, map_period)
+        elif isinstance(interp, AnalyticalInterpolator):
+            field, k = interp.gridded().device_field()
             ftle = W.advect_ftle(field, init, method=method, time_mode="steady", n_steps=num_steps,
                                  h=timestep, map_period=map_period, first=k, positions_out=dev,
                                  cfg=self.launch)
+            interp.check_inside_grid(dev)
             for i in range(num_steps):
                 self.source.update_interpolator(interp, i)
                 if self.progress_queue:

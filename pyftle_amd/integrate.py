@@ -8,6 +8,12 @@ a window (``advect``).
 
 AB2 keeps its previous velocity on the integrator (integrate.py:220,245-257):
 the first call is an Euler step, later calls use ``1.5 v - 0.5 v_prev``.
+
+With an ``AnalyticalInterpolator`` (and no opt-in grid) a step is taken the
+reference's way: the user's ``velocity_fn`` is evaluated on the device stage
+positions at every stage (k1..k4 for RK4), and ``pyftle_stage_combine`` forms
+the stage positions and the update in the reference's operation order
+(integrate.py:16-109, 291-342).
 """
 
 from __future__ import annotations
@@ -47,16 +53,49 @@ class Integrator(ABC):
     def __init__(self, interpolator: Interpolator, **kwargs) -> None:
         self.interpolator = interpolator
 
-    def _grid(self, particles) -> HighPerfInterpolator:
+    def _grid(self) -> HighPerfInterpolator:
         interp = self.interpolator
         if isinstance(interp, AnalyticalInterpolator):
-            return interp.gridded(particles.positions)
+            return interp.gridded()
         if not isinstance(interp, HighPerfInterpolator):
             raise TypeError("device integrators need a 'grid' or 'analytical' interpolator")
         return interp
 
+    def stage_evaluated(self) -> bool:
+        """True when steps call the analytical velocity function at every stage."""
+        interp = self.interpolator
+        return isinstance(interp, AnalyticalInterpolator) and not interp.is_gridded
+
+    def _combine(self, op: int, dev: torch.Tensor, c: float, k1, k2=None, k3=None, k4=None, out=None):
+        N.call("pyftle_stage_combine", op, dtype_code(dev.dtype), dev.numel(), float(c), ptr(dev),
+               ptr(k1), ptr(k2) if k2 is not None else None, ptr(k3) if k3 is not None else None,
+               ptr(k4) if k4 is not None else None, ptr(out) if out is not None else None, stream_handle())
+
+    def step_device(self, h: float, dev: torch.Tensor) -> None:
+        """One step of the analytical strategy on a contiguous device tensor, in place."""
+        raise NotImplementedError
+
+    def _device_positions(self, particles):
+        pos = particles.positions
+        on_host = not isinstance(pos, torch.Tensor)
+        dev = to_device(pos, torch.float64 if on_host else None)
+        if dev.dtype not in (torch.float32, torch.float64):
+            dev = dev.to(torch.float64)
+        return pos, on_host, dev.contiguous()
+
+    def _write_back(self, pos, on_host, dev) -> None:
+        if on_host:
+            np.copyto(pos, dev.cpu().numpy())
+        elif dev.data_ptr() != pos.data_ptr():
+            pos.copy_(dev)
+
     def _step(self, h: float, particles, vprev=None, have_prev=False) -> None:
-        grid = self._grid(particles)
+        if self.stage_evaluated():
+            pos, on_host, dev = self._device_positions(particles)
+            self.step_device(h, dev)
+            self._write_back(pos, on_host, dev)
+            return
+        grid = self._grid()
         field, k = grid.device_field()
         pos = particles.positions
         on_host = not isinstance(pos, torch.Tensor)
@@ -85,6 +124,10 @@ class EulerIntegrator(Integrator):
     def integrate(self, h: float, particles) -> None:
         self._step(h, particles)
 
+    def step_device(self, h: float, dev: torch.Tensor) -> None:
+        v = self.interpolator.evaluate(dev)
+        self._combine(N.COMBINE_EULER, dev, h, v)
+
 
 class AdamsBashforth2Integrator(Integrator):
     """integrate.py:204-257: Euler first, then ``x += h (1.5 v - 0.5 v_prev)``."""
@@ -96,6 +139,9 @@ class AdamsBashforth2Integrator(Integrator):
         self.previous_velocity = None  # device tensor, same shape as positions
 
     def integrate(self, h: float, particles) -> None:
+        if self.stage_evaluated():
+            self._step(h, particles)
+            return
         pos = particles.positions
         dt = pos.dtype if isinstance(pos, torch.Tensor) and pos.dtype in (torch.float32, torch.float64) \
             else torch.float64
@@ -104,14 +150,42 @@ class AdamsBashforth2Integrator(Integrator):
             self.previous_velocity = torch.zeros(tuple(pos.shape), dtype=dt, device=torch.device("cuda"))
         self._step(h, particles, vprev=self.previous_velocity, have_prev=have_prev)
 
+    def step_device(self, h: float, dev: torch.Tensor) -> None:
+        v = self.interpolator.evaluate(dev)
+        prev = self.previous_velocity
+        if prev is None or tuple(prev.shape) != tuple(dev.shape) or prev.dtype != dev.dtype:
+            # first call: Euler, then keep v (integrate.py:248-257)
+            self.previous_velocity = torch.empty_like(dev)
+            self._combine(N.COMBINE_EULER, dev, h, v, out=self.previous_velocity)
+        else:
+            self._combine(N.COMBINE_AB2, dev, h, v, prev, out=prev)
+
 
 class RungeKutta4Integrator(Integrator):
     """integrate.py:265-342: classic RK4, field frozen across the stages."""
 
     method = N.RK4
 
+    def __init__(self, interpolator: Interpolator, **kwargs) -> None:
+        super().__init__(interpolator, **kwargs)
+        self._tmp = None  # stage-position buffer of the analytical path (integrate.py:316-321)
+
     def integrate(self, h: float, particles) -> None:
         self._step(h, particles)
+
+    def step_device(self, h: float, dev: torch.Tensor) -> None:
+        f = self.interpolator.evaluate
+        if self._tmp is None or self._tmp.shape != dev.shape or self._tmp.dtype != dev.dtype:
+            self._tmp = torch.empty_like(dev)
+        tmp = self._tmp
+        k1 = f(dev)
+        self._combine(N.COMBINE_STAGE, dev, 0.5 * h, k1, out=tmp)
+        k2 = f(tmp)
+        self._combine(N.COMBINE_STAGE, dev, 0.5 * h, k2, out=tmp)
+        k3 = f(tmp)
+        self._combine(N.COMBINE_STAGE, dev, h, k3, out=tmp)
+        k4 = f(tmp)
+        self._combine(N.COMBINE_RK4, dev, h / 6.0, k1, k2, k3, k4)
 
 
 INTEGRATORS = {

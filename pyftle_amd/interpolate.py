@@ -355,10 +355,20 @@ class NearestNeighborInterpolator(_ScatteredInterpolator):
 class AnalyticalInterpolator(Interpolator):
     """interpolate.py:765-829: ``velocity_fn(time, points)`` with time fixed at 0.
 
-    ``interpolate`` evaluates the user's host function exactly as the
-    reference does.  The device integrators cannot call a host function, so
-    they use ``gridded()``: the field sampled at ``self.time`` onto a uniform
-    grid (interpolation error O(spacing^2)).
+    The device integrators call the function exactly where the reference does: at
+    every integrator stage, with ``time`` (0.0) and the ``(M, dim)`` stage positions,
+    here a CUDA tensor of the positions' dtype (FP64 for the reference's arrays).  It
+    must return the ``(M, dim)`` velocities on that device, so it is written with
+    torch operations (``torch.sin``, ``torch.exp`` ...); a function built on NumPy
+    ufuncs cannot take device tensors and raises ``TypeError`` saying so.  Global
+    reductions inside the function (a normalisation by the largest speed, as in the
+    reference's helical-vortex example) therefore see the same point set as in the
+    reference.  The HIP stage kernels (``pyftle_stage_combine``) combine the stages.
+
+    Opt-in: ``set_grid(lower, upper, shape)`` samples the function once, at
+    ``time``, onto a uniform device grid that the fused window kernel reads
+    (bilinear / trilinear error O(spacing^2) of the field's second derivatives);
+    a particle that leaves the sampled box makes the window raise ``ValueError``.
     """
 
     def __init__(self, velocity_fn: Callable):
@@ -371,29 +381,53 @@ class AnalyticalInterpolator(Interpolator):
     def interpolate(self, new_points):
         if not callable(self.velocity_fn):
             raise ExecutableNotFoundError("velocity_fn was not assigned properly")
+        if isinstance(new_points, torch.Tensor) and new_points.is_cuda:
+            return self.evaluate(new_points)
         return self.velocity_fn(self.time, new_points)
+
+    def evaluate(self, points: torch.Tensor) -> torch.Tensor:
+        """``velocity_fn(time, points)`` on a device tensor, checked and made a
+        contiguous tensor of the points' dtype and shape."""
+        if not callable(self.velocity_fn):
+            raise ExecutableNotFoundError("velocity_fn was not assigned properly")
+        try:
+            v = self.velocity_fn(self.time, points)
+        except TypeError as e:
+            raise TypeError(
+                "velocity_fn(t, points) is called with a CUDA tensor on the device path and must "
+                "return one (write it with torch operations, e.g. torch.sin instead of np.sin), "
+                "or sample it onto a device grid with set_grid() / AnalyticalSolver(grid=...)"
+            ) from e
+        if not isinstance(v, torch.Tensor):
+            v = torch.as_tensor(np.asarray(v), device=points.device)
+        if tuple(v.shape) != tuple(points.shape):
+            raise ValueError(f"velocity_fn returned shape {tuple(v.shape)} for points of shape "
+                             f"{tuple(points.shape)}")
+        if v.device != points.device or v.dtype != points.dtype:
+            v = v.to(device=points.device, dtype=points.dtype)
+        v = v.contiguous()
+        if v.data_ptr() == points.data_ptr():  # the stage buffer is rewritten before v is used
+            v = v.clone()
+        return v
 
     def update(self, velocities, points=None) -> None:
         pass
 
+    @property
+    def is_gridded(self) -> bool:
+        return self._grid_spec is not None
+
     def set_grid(self, lower, upper, shape, precision="float32", periodic=None) -> None:
-        """Domain and resolution used when the field is evaluated on the device."""
+        """Opt in to evaluating the field once on a uniform device grid over
+        ``[lower, upper]`` with ``shape`` nodes per axis."""
         self._grid_spec = (tuple(lower), tuple(upper), tuple(int(s) for s in shape), precision, periodic)
         self._grid = None
 
-    def gridded(self, particles_positions=None) -> HighPerfInterpolator:
+    def gridded(self) -> HighPerfInterpolator:
         if self._grid is not None:
             return self._grid
         if self._grid_spec is None:
-            if particles_positions is None:
-                raise ValueError("AnalyticalInterpolator needs set_grid() before device use")
-            p = np.asarray(particles_positions.detach().cpu() if isinstance(particles_positions, torch.Tensor)
-                           else particles_positions, dtype=np.float64)
-            lo, hi = p.min(axis=0), p.max(axis=0)
-            pad = 0.25 * (hi - lo) + 1e-6
-            dim = p.shape[1]
-            shape = (1024, 1024) if dim == 2 else (128, 128, 128)
-            self._grid_spec = (tuple(lo - pad), tuple(hi + pad), shape, "float32", None)
+            raise ValueError("AnalyticalInterpolator.gridded() needs set_grid() first")
         lower, upper, shape, precision, periodic = self._grid_spec
         axes = [np.linspace(lower[a], upper[a], shape[a]) for a in range(len(shape))]
         if len(shape) == 2:
@@ -402,11 +436,32 @@ class AnalyticalInterpolator(Interpolator):
         else:
             X, Y, Z = np.meshgrid(*axes)  # 'xy' meshgrid, flat C order (interpolate.py:383-385)
             pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
-        vel = np.asarray(self.velocity_fn(self.time, pts), dtype=np.float64)
+        dev_pts = torch.as_tensor(pts, device="cuda")
+        try:
+            vel = self.evaluate(dev_pts).to(torch.float64)
+        except TypeError:
+            # a NumPy-only function: sampling the grid is ingest (like reading a snapshot
+            # file), done once on the host; the window itself runs on the device
+            vel = torch.as_tensor(np.asarray(self.velocity_fn(self.time, pts), dtype=np.float64),
+                                  device="cuda")
         g = HighPerfInterpolator(shape, precision=precision, periodic=periodic)
-        g.update(np.ascontiguousarray(vel.T), np.ascontiguousarray(pts.T))
+        g.update(vel.t().contiguous(), dev_pts.t().contiguous())
         self._grid = g
         return g
+
+    def check_inside_grid(self, positions: torch.Tensor) -> None:
+        """Raises ``ValueError`` if a particle lies outside the sampled box (where the
+        gridded field is zero and the particle would have stopped)."""
+        lower, upper, shape, _, periodic = self._grid_spec
+        lo = torch.as_tensor(lower, dtype=positions.dtype, device=positions.device)
+        hi = torch.as_tensor(upper, dtype=positions.dtype, device=positions.device)
+        out = (positions < lo) | (positions > hi)
+        if periodic is not None:
+            out &= ~torch.as_tensor([bool(p) for p in periodic], device=positions.device)
+        n_out = int(out.any(dim=1).sum().item())
+        if n_out:
+            raise ValueError(f"{n_out} particles left the sampled box [{lower}, {upper}] of the gridded "
+                             "analytical field; enlarge the grid passed to set_grid()")
 
 
 def create_interpolator(interpolation_type: str, grid_shape=None, velocity_fn=None, **kwargs) -> Interpolator:

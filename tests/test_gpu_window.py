@@ -331,3 +331,77 @@ def test_analytical_solver_gridded_field_converges_to_exact_oracle():
         errs.append(np.max(np.abs(got - exp_ftle)) / rng)
     assert errs[0] <= 1e-3
     assert errs[1] <= 0.35 * errs[0]
+
+
+@pytest.mark.parametrize("method", ["rk4", "euler", "ab2"])
+def test_analytical_solver_default_evaluates_function_at_every_stage(method):
+    """AnalyticalSolver (api.py:15-224) with its defaults: velocity_fn(0.0, points) on the
+    device stage positions at every stage (interpolate.py:788,812; integrate.py:291-342),
+    against the oracle's exact-function window (run_window_reference with the NumPy
+    double gyre).  FP64; the two sin/cos implementations differ by ulps only."""
+    from oracle import flows
+    from pyftle_amd import NeighboringParticles
+    from pyftle_amd.api import AnalyticalSolver
+
+    x0 = H.seeds_2d(40, 20)
+    fn = lambda x: flows.double_gyre(0.0, x)  # noqa: E731
+    exp_ftle, exp_pos = O.run_window_reference([fn] * 11, x0, 0.1, method)
+    parts = NeighboringParticles(x0.copy())
+    s = AnalyticalSolver(H.torch_double_gyre, parts, 0.1, 1.0, 1, method)
+    got = s.run()
+    rng = np.nanmax(exp_ftle) - np.nanmin(exp_ftle)
+    err = np.max(np.abs(got - exp_ftle)) / rng
+    print(f"[analytical {method}] max|err|/range = {err:.3e}")
+    assert err <= 1e-10
+
+
+def test_analytical_solver_windows_and_backward_helical():
+    """The reference's helical-vortex example (examples/01_Closed-form_velocity_helical.ipynb),
+    scaled down to 8x8x16 seeds: backward time (dt = -0.01, T = 0.02 -> 3 steps), Euler as in
+    the notebook and RK4, two windows.  The function normalises by the largest speed over the
+    queried points, so it must see the stage positions exactly as in the reference."""
+    from oracle import flows
+    from pyftle_amd import NeighboringParticles
+    from pyftle_amd.api import AnalyticalSolver
+
+    x0 = H.helical_particles(8, 8, 16)
+    fn = lambda x: flows.helical_vortex(0.0, x)  # noqa: E731
+    for method in ("euler", "rk4"):
+        exp_ftle, _ = O.run_window_reference([fn] * 3, x0, -0.01, method)
+        s = AnalyticalSolver(H.torch_helical_vortex, NeighboringParticles(x0.copy()), -0.01, 0.02, 2, method)
+        got = s.run()
+        assert got.shape == (2, exp_ftle.shape[0])
+        rng = np.nanmax(exp_ftle) - np.nanmin(exp_ftle)
+        for w in range(2):  # every window starts from the original particles at frozen time 0
+            err = np.max(np.abs(got[w] - exp_ftle)) / rng
+            print(f"[helical {method} window {w}] max|err|/range = {err:.3e}")
+            assert err <= 1e-10
+
+
+def test_analytical_integrator_step_and_errors():
+    """Integrator.integrate(h, particles) on host positions with the analytical strategy
+    equals one oracle step; a NumPy-only function raises TypeError on the device path; the
+    opt-in grid raises when a particle leaves the sampled box."""
+    from oracle import flows
+    from pyftle_amd import NeighboringParticles
+    from pyftle_amd.api import AnalyticalSolver
+    from pyftle_amd.integrate import create_integrator
+    from pyftle_amd.interpolate import create_interpolator
+
+    x0 = H.seeds_2d(6, 4)
+    integ = create_integrator("rk4", create_interpolator("analytical", velocity_fn=H.torch_double_gyre))
+    parts = NeighboringParticles(x0.copy())
+    integ.integrate(0.1, parts)
+    exp = x0.copy()
+    O.rk4_step(exp, 0.1, lambda x: flows.double_gyre(0.0, x))
+    np.testing.assert_allclose(parts.positions, exp, rtol=0, atol=1e-14)
+    # AnalyticalSolver.run reports a failed window as RuntimeError (parallel.py:236-269)
+    with pytest.raises(RuntimeError) as ei:
+        AnalyticalSolver(flows.double_gyre, NeighboringParticles(x0.copy()), 0.1, 0.2, 1, "rk4").run()
+    assert isinstance(ei.value.__cause__, TypeError) and "torch" in str(ei.value.__cause__)
+    far = NeighboringParticles(x0.copy())
+    s = AnalyticalSolver(lambda t, q: 0 * q + 1.0, far, 0.1, 1.0, 1, "euler",
+                         grid=((0.0, 0.0), (2.0, 1.0), (64, 32)), precision="float64")
+    with pytest.raises(RuntimeError) as ei:
+        s.run()
+    assert isinstance(ei.value.__cause__, ValueError) and "left the sampled box" in str(ei.value.__cause__)
