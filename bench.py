@@ -1099,16 +1099,47 @@ def main():
     value = n_seeds * args.steps / wall
     seed_steps = n_seeds * wl["steps"]
     alg_bytes = wl["bytes_per_seed_step"] * seed_steps
-    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    achieved_alg = alg_bytes / (avg_ms * 1e-3) / 1e9
     traffic = load_traffic(wl["name"])
-    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic.get("hbm_bytes_per_launch") if traffic else None,
-            "kernel": (f"advect_kernel_cell<3,RK4,STEADY,FTLE> ({_records_name(3)} records)" if wl["dim"] == 3
-                       else f"advect_kernel_cell<2,RK4,LINEAR,PAIR,FTLE> ({_records_name(2)} records)"),
-            "algorithmic_bytes_per_launch": alg_bytes,
-            "bytes_per_seed_step": wl["bytes_per_seed_step"],
-            "avg_launch_ms": avg_ms}
+    hbm_bytes = traffic.get("hbm_bytes_per_launch") if traffic else None
+    hbm = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "algorithmic_bytes_per_launch": alg_bytes, "bytes_per_seed_step": wl["bytes_per_seed_step"],
+           "achieved_algorithmic": achieved_alg, "frac_algorithmic": achieved_alg / HBM_PEAK_GBS,
+           "measured_bytes_per_launch": hbm_bytes,
+           "measured": hbm_bytes / (avg_ms * 1e-3) / 1e9 if hbm_bytes else None,
+           "frac_measured": hbm_bytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if hbm_bytes else None}
+    kernel_name = (f"advect_kernel_cell<3,RK4,STEADY,FTLE> ({_records_name(3)} records)" if wl["dim"] == 3
+                   else f"advect_kernel_cell<2,RK4,LINEAR,PAIR,FTLE> ({_records_name(2)} records)")
+    if _records_name(wl["dim"]) == "cell":
+        # The binding unit: the vector-memory (texture) path that serves the record gathers.
+        # Issued wave-level 16-byte record loads of one launch, counted by a copy of the kernel
+        # (same inputs, launch shape and seed order; results identical), times 1 KiB per wave
+        # load, over the timed launch time; peak = the same path's rate for coalesced L1-resident
+        # 16-byte wave loads, measured now by pyftle_probe_load_rate.
+        log("counting issued record loads; probing the vector-memory peak")
+        cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+        ftle_c = torch.empty_like(ftle)
+        W.advect_ftle(wl["field"], wl["pos"], method="rk4", time_mode=wl["time_mode"], n_steps=wl["steps"],
+                      h=wl["h"], t_start=wl["t_start"], map_period=wl["map_period"], ftle_out=ftle_c, cfg=cfg,
+                      order=order(), count_loads=cnt)
+        torch.cuda.synchronize()
+        loads = int(cnt.item())
+        probe = W.probe_load_rate()
+        issued = loads * 1024 / (avg_ms * 1e-3) / 1e9
+        peak = probe["bytes_per_s"] / 1e9
+        roof = {"bound": "td", "achieved": issued, "peak": peak, "unit": "GB/s", "frac": issued / peak,
+                "traffic": hbm_bytes, "kernel": kernel_name, "avg_launch_ms": avg_ms,
+                "issued_wave_loads_per_launch": loads, "bytes_per_wave_load": 1024,
+                "counting_launch_identical": bool(torch.equal(ftle_c, ftle)),
+                "peak_probe": {"kernel": "load_rate_kernel (coalesced, L1-resident 16-byte wave loads)",
+                               "wave_loads": probe["wave_loads"], "ms": probe["seconds"] * 1e3,
+                               "wave_loads_per_s": probe["wave_loads_per_s"]},
+                "hbm": hbm}
+    else:
+        roof = {"bound": "hbm", "achieved": achieved_alg, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved_alg / HBM_PEAK_GBS, "traffic": hbm_bytes, "kernel": kernel_name,
+                "algorithmic_bytes_per_launch": alg_bytes, "bytes_per_seed_step": wl["bytes_per_seed_step"],
+                "avg_launch_ms": avg_ms}
 
     result = {
         "metric": METRIC,

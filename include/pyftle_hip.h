@@ -257,6 +257,23 @@ int pyftle_advect_ftle(const pyftle_field *field, const pyftle_advect_opts *opts
                        const void *positions_in, int32_t pos_dtype, int64_t n_groups,
                        void *positions_out, double map_period, double *ftle_out, void *stream);
 
+/* ---- instrumentation (no reference counterpart): measuring the gather-bound kernel ----
+ * pyftle_advect_ftle_count_loads: the same window as pyftle_advect_ftle (identical
+ * results) for RK4 on FP32 CELL / CELL_PAIR records, launched with a counting copy of
+ * the kernel that adds the wave-level 16-byte record loads it issues to *loads (device
+ * uint64, accumulated; a wave issues a record's loads when any of its lanes needs the
+ * record).  Run outside timed regions: the count is a property of the inputs and the
+ * launch shape, the timed launches use the plain kernel.
+ * pyftle_probe_load_rate: `blocks` work-groups of 256 lanes each issue `iters` (a
+ * multiple of 4) 16-byte-per-lane wave loads, coalesced and L1-resident, from the first
+ * 16 KiB of `buf`; one float per lane goes to `out` (blocks * 256).  Its rate is the peak
+ * of the vector-memory path for such loads. */
+int pyftle_advect_ftle_count_loads(const pyftle_field *field, const pyftle_advect_opts *opts,
+                                   const void *positions_in, int32_t pos_dtype, int64_t n_groups,
+                                   void *positions_out, double map_period, double *ftle_out,
+                                   unsigned long long *loads, void *stream);
+int pyftle_probe_load_rate(const void *buf, int32_t blocks, int32_t iters, float *out, void *stream);
+
 /* ---- FTLE tensors (FP64 in/out) ----
  * pyftle_flow_map_jacobian: cauchy_green.py:15-153.  positions / initial
  *   positions block-ordered (nb*n, dim) of pos_dtype -> J (n,dim,dim).

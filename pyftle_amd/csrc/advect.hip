@@ -139,9 +139,14 @@ int pyftle_advect(const pyftle_field *field, const pyftle_advect_opts *opts, voi
   return check_launch("advect");
 }
 
-int pyftle_advect_ftle(const pyftle_field *field, const pyftle_advect_opts *opts,
-                       const void *positions_in, int32_t pos_dtype, int64_t n_groups,
-                       void *positions_out, double map_period, double *ftle_out, void *stream) {
+}  // extern "C"
+
+namespace pyftle {
+
+static int advect_ftle_impl(const pyftle_field *field, const pyftle_advect_opts *opts,
+                            const void *positions_in, int32_t pos_dtype, int64_t n_groups,
+                            void *positions_out, double map_period, double *ftle_out,
+                            unsigned long long *load_count, void *stream) {
   AdvectArgs a{};
   int rc = make_field_dev(field, &a.f);
   if (rc) return rc;
@@ -173,15 +178,41 @@ int pyftle_advect_ftle(const pyftle_field *field, const pyftle_advect_opts *opts
   a.h = opts->h;
   a.map_period = map_period;
   a.order = opts->seed_order;
+  a.load_count = load_count;
   const int64_t blocks = (n_groups + G - 1) / G;
   if (blocks > INT32_MAX) return fail("advect_ftle: too many seeds");
   int threads = G * nb;
   threads = ((threads + 63) / 64) * 64;
   LaunchSpec sp{opts->method, opts->time_mode, layout_pair(field->layout),
                 field->dtype == PYFTLE_F64, prec_of(opts), 1, (unsigned)blocks, (unsigned)threads,
-                as_stream(stream), layout_cell(field->layout)};
+                as_stream(stream), layout_cell(field->layout), load_count ? 1 : 0};
   dispatch(field, a, sp);
   return check_launch("advect_ftle");
 }
+
+}  // namespace pyftle
+
+extern "C" {
+
+int pyftle_advect_ftle(const pyftle_field *field, const pyftle_advect_opts *opts,
+                       const void *positions_in, int32_t pos_dtype, int64_t n_groups,
+                       void *positions_out, double map_period, double *ftle_out, void *stream) {
+  return advect_ftle_impl(field, opts, positions_in, pos_dtype, n_groups, positions_out, map_period,
+                          ftle_out, nullptr, stream);
+}
+
+int pyftle_advect_ftle_count_loads(const pyftle_field *field, const pyftle_advect_opts *opts,
+                                   const void *positions_in, int32_t pos_dtype, int64_t n_groups,
+                                   void *positions_out, double map_period, double *ftle_out,
+                                   unsigned long long *loads, void *stream) {
+  if (!loads) return fail("advect_ftle_count_loads: loads is NULL");
+  if (!field || !opts) return fail("advect_ftle_count_loads: NULL field or opts");
+  if (!layout_cell(field->layout) || field->dtype != PYFTLE_F32 || field->interp != PYFTLE_INTERP_GINTERP ||
+      opts->method != PYFTLE_RK4 || opts->compute != PYFTLE_COMPUTE_NATIVE)
+    return fail("advect_ftle_count_loads: counts RK4 windows of the FP32 kernel on CELL / CELL_PAIR records");
+  return advect_ftle_impl(field, opts, positions_in, pos_dtype, n_groups, positions_out, map_period,
+                          ftle_out, loads, stream);
+}
+
 
 }  // extern "C"

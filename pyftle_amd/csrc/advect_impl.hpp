@@ -47,6 +47,7 @@ struct AdvectArgs {
   double h;
   double map_period;
   const int *order;  // processing order of the seed groups (NULL: identity)
+  unsigned long long *load_count;  // counting launches: wave-level record loads issued (else NULL)
 };
 
 // index of the k-th processed seed group (an entry outside [0, n_groups) maps to k)
@@ -397,11 +398,13 @@ __device__ __forceinline__ int cell_axis(const FieldDev &f, int a, int cell, flo
 // launch a record's contents depend on (slot, index) alone, so the results are
 // the same bit for bit.  Steady fields pass slot 0 throughout (a constant the
 // compiler folds away); the others start from slot -1, which no sample has.
-template <int DIM, bool PAIR, bool CELL, bool CACHE = false, bool SLOTS = false>
+// COUNT adds the wave-level load instructions the sample issues to *nload
+// (uniform over the wave: a wave issues a record's loads when any lane misses).
+template <int DIM, bool PAIR, bool CELL, bool CACHE = false, bool SLOTS = false, bool COUNT = false>
 __device__ __forceinline__ void sample_cell(const FieldDev &f, __amdgpu_buffer_rsrc_t rs, float w,
                                             const int *cell, const float *t, float *vel,
                                             int *ckey = nullptr, int *cslot = nullptr,
-                                            float *crec = nullptr, int slot = 0) {
+                                            float *crec = nullptr, int slot = 0, int *nload = nullptr) {
   constexpr int C = RecInfo<DIM, PAIR>::comps;
   constexpr int L = RecInfo<DIM, PAIR>::per_level;
   constexpr int NF = C > 4 ? 4 : C;
@@ -425,7 +428,9 @@ __device__ __forceinline__ void sample_cell(const FieldDev &f, __amdgpu_buffer_r
     float local[RF];
     float *rec = CACHE ? crec : local;
     const unsigned roff = (unsigned)base * (unsigned)(RF * 4);  // out-of-range offsets read 0
-    if (!CACHE || base != *ckey || (SLOTS && slot != *cslot)) {
+    const bool miss = !CACHE || base != *ckey || (SLOTS && slot != *cslot);
+    if constexpr (COUNT) *nload += __builtin_amdgcn_ballot_w64(miss) ? RF / 4 : 0;
+    if (miss) {
 #pragma unroll
       for (int h = 0; h < RF / 4; ++h) BufVec<4>::load(rs, (int)(roff + 16u * h), &rec[h * 4]);
       if constexpr (CACHE) *ckey = base;
@@ -438,6 +443,7 @@ __device__ __forceinline__ void sample_cell(const FieldDev &f, __amdgpu_buffer_r
 #pragma unroll
         for (int comp = 0; comp < DIM; ++comp) c[q][l * L + comp] = rec[(l * NQ + q) * DIM + comp];
   } else {
+    if constexpr (COUNT) *nload += NQ * chunks;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int node = base + ((q & 1) ? 1 : 0) + ((q & 2) ? f.px : 0) + ((q & 4) ? f.pxy : 0);
@@ -485,7 +491,7 @@ __device__ __forceinline__ int wrap_cell(int &cell, int n, float inv_n) {
 }
 
 // One work-group's share of a window: logical block lb of the seeds of `a`.
-template <int DIM, int METHOD, int TMODE, bool PAIR, bool FTLE, bool CELL>
+template <int DIM, int METHOD, int TMODE, bool PAIR, bool FTLE, bool CELL, bool COUNT = false>
 __device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t lb) {
   const int t = threadIdx.x;
   const int nb = a.nb, G = a.G;
@@ -554,6 +560,7 @@ __device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t l
   constexpr bool CACHE = CELL;
   constexpr int CRF = CACHE ? (1 << DIM) * DIM * (PAIR ? 2 : 1) : 1;
   int ckey = -1, cslot = -1;
+  int nload = 0;  // COUNT: wave-level record loads issued by this wave
   float crec[CRF];
 #pragma unroll
   for (int i = 0; i < CRF; ++i) crec[i] = 0.0f;
@@ -579,18 +586,18 @@ __device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t l
     double inc[DIM];
 #pragma unroll
     for (int c = 0; c < DIM; ++c) tp[c] = frac[c];
-    sample_cell<DIM, PAIR, CELL, CACHE, TMODE != PYFTLE_TIME_STEADY>(f, r0, (float)w0, cell, tp, k1, &ckey, &cslot, crec, q0);
+    sample_cell<DIM, PAIR, CELL, CACHE, TMODE != PYFTLE_TIME_STEADY, COUNT>(f, r0, (float)w0, cell, tp, k1, &ckey, &cslot, crec, q0, &nload);
     if (METHOD == PYFTLE_RK4) {
       float k2[DIM], k3[DIM], k4[DIM];
 #pragma unroll
       for (int c = 0; c < DIM; ++c) tp[c] = fmaf(sh[c], k1[c], frac[c]);
-      sample_cell<DIM, PAIR, CELL, CACHE, TMODE != PYFTLE_TIME_STEADY>(f, r1, (float)w1, cell, tp, k2, &ckey, &cslot, crec, q1);
+      sample_cell<DIM, PAIR, CELL, CACHE, TMODE != PYFTLE_TIME_STEADY, COUNT>(f, r1, (float)w1, cell, tp, k2, &ckey, &cslot, crec, q1, &nload);
 #pragma unroll
       for (int c = 0; c < DIM; ++c) tp[c] = fmaf(sh[c], k2[c], frac[c]);
-      sample_cell<DIM, PAIR, CELL, CACHE, TMODE != PYFTLE_TIME_STEADY>(f, r1, (float)w1, cell, tp, k3, &ckey, &cslot, crec, q1);
+      sample_cell<DIM, PAIR, CELL, CACHE, TMODE != PYFTLE_TIME_STEADY, COUNT>(f, r1, (float)w1, cell, tp, k3, &ckey, &cslot, crec, q1, &nload);
 #pragma unroll
       for (int c = 0; c < DIM; ++c) tp[c] = fmaf(sf[c], k3[c], frac[c]);
-      sample_cell<DIM, PAIR, CELL, CACHE, TMODE != PYFTLE_TIME_STEADY>(f, r2, (float)w2, cell, tp, k4, &ckey, &cslot, crec, q2);
+      sample_cell<DIM, PAIR, CELL, CACHE, TMODE != PYFTLE_TIME_STEADY, COUNT>(f, r2, (float)w2, cell, tp, k4, &ckey, &cslot, crec, q2, &nload);
 #pragma unroll
       for (int c = 0; c < DIM; ++c)
         inc[c] = s6d[c] * ((double)k1[c] + 2.0 * ((double)k2[c] + (double)k3[c]) + (double)k4[c]);
@@ -671,11 +678,14 @@ __device__ __forceinline__ void advect_block_cell(const AdvectArgs &a, int64_t l
     }
   }
   if constexpr (FTLE) ftle_epilogue<DIM>(a, lb, t, sl, j, fin, x0);
+  if constexpr (COUNT) {
+    if ((threadIdx.x & 63) == 0 && a.load_count) atomicAdd(a.load_count, (unsigned long long)nload);
+  }
 }
 
-template <int DIM, int METHOD, int TMODE, bool PAIR, bool FTLE, bool CELL>
+template <int DIM, int METHOD, int TMODE, bool PAIR, bool FTLE, bool CELL, bool COUNT = false>
 __global__ void __launch_bounds__(512) advect_kernel_cell(AdvectArgs a) {
-  advect_block_cell<DIM, METHOD, TMODE, PAIR, FTLE, CELL>(a, logical_block(a.remap));
+  advect_block_cell<DIM, METHOD, TMODE, PAIR, FTLE, CELL, COUNT>(a, logical_block(a.remap));
 }
 
 // ---------------------------------------------------------------------------
@@ -686,12 +696,19 @@ struct LaunchSpec {
   int method, tmode, pair, store64, prec, ftle;
   unsigned blocks, threads;
   hipStream_t stream;
-  int cell;  // per-cell records (FP32, PREC_NATIVE only)
+  int cell;   // per-cell records (FP32, PREC_NATIVE only)
+  int count;  // counting launch (RK4 FTLE windows on per-cell records; AdvectArgs.load_count)
 };
 
 template <int DIM, int METHOD, int TMODE, bool PAIR, bool STORE64, int PREC>
 void launch_leaf(const AdvectArgs &a, const LaunchSpec &sp) {
   if constexpr (!STORE64 && PREC == PREC_NATIVE) {
+    if constexpr (METHOD == PYFTLE_RK4) {
+      if (sp.count) {  // the caller checked: per-cell records, FTLE window
+        advect_kernel_cell<DIM, METHOD, TMODE, PAIR, true, true, true><<<sp.blocks, sp.threads, 0, sp.stream>>>(a);
+        return;
+      }
+    }
     if (sp.cell) {
       if (sp.ftle)
         advect_kernel_cell<DIM, METHOD, TMODE, PAIR, true, true><<<sp.blocks, sp.threads, 0, sp.stream>>>(a);

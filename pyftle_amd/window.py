@@ -92,11 +92,15 @@ def tuned_config(field: DeviceField, n_groups: int, time_mode) -> LaunchConfig:
 def advect_ftle(field: DeviceField, positions: torch.Tensor, *, method="rk4", time_mode="frozen",
                 n_steps: int, h: float, map_period: float, first: int = 0, t_start: float = 0.0,
                 positions_out: torch.Tensor | None = None, ftle_out: torch.Tensor | None = None,
-                cfg: LaunchConfig | None = None, stream=None, order=None) -> torch.Tensor:
+                cfg: LaunchConfig | None = None, stream=None, order=None,
+                count_loads: torch.Tensor | None = None) -> torch.Tensor:
     """Advect block-ordered seed groups and return the FP64 FTLE per seed.
 
     ``order``: ``None``, ``"morton"`` (``seed_order``) or an int32 device permutation of the
-    seed groups giving the order in which they are processed (results are the same)."""
+    seed groups giving the order in which they are processed (results are the same).
+    ``count_loads``: a one-element int64 device tensor; the launch then uses the counting
+    copy of the kernel (``pyftle_advect_ftle_count_loads``, same results) and adds the
+    wave-level record loads it issues (RK4 on per-cell FP32 records only)."""
     if not positions.is_cuda:
         raise ValueError("positions must be a device tensor")
     positions = positions.contiguous()
@@ -122,10 +126,39 @@ def advect_ftle(field: DeviceField, positions: torch.Tensor, *, method="rk4", ti
         order = order.contiguous()
         opts.seed_order = order.data_ptr()
     pout = ptr(positions_out) if positions_out is not None else ctypes.c_void_p(0)
+    if count_loads is not None:
+        if count_loads.dtype != torch.int64 or not count_loads.is_cuda or count_loads.numel() != 1:
+            raise ValueError("count_loads must be a one-element int64 device tensor")
+        N.call("pyftle_advect_ftle_count_loads", ctypes.byref(field.desc), ctypes.byref(opts), ptr(positions),
+               dtype_code(positions.dtype), n_groups, pout, float(map_period), ptr(ftle_out),
+               ptr(count_loads), stream_handle(stream))
+        return ftle_out
     N.call("pyftle_advect_ftle", ctypes.byref(field.desc), ctypes.byref(opts), ptr(positions),
            dtype_code(positions.dtype), n_groups, pout, float(map_period), ptr(ftle_out),
            stream_handle(stream))
     return ftle_out
+
+
+def probe_load_rate(blocks: int = 2048, iters: int = 8192, repeats: int = 5, stream=None) -> dict:
+    """Peak rate of 16-byte-per-lane wave loads through the vector-memory path
+    (``pyftle_probe_load_rate``: coalesced, L1-resident), best of ``repeats`` launches
+    timed with HIP events on the launch stream."""
+    buf = torch.zeros(4096, dtype=torch.float32, device="cuda")
+    out = torch.empty(blocks * 256, dtype=torch.float32, device="cuda")
+    st = stream or torch.cuda.current_stream()
+    N.call("pyftle_probe_load_rate", ptr(buf), int(blocks), int(iters), ptr(out), stream_handle(st))
+    best = float("inf")
+    for _ in range(repeats):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        N.call("pyftle_probe_load_rate", ptr(buf), int(blocks), int(iters), ptr(out), stream_handle(st))
+        b.record(st)
+        b.synchronize()
+        best = min(best, a.elapsed_time(b) * 1e-3)
+    wave_loads = blocks * 4 * iters
+    return {"wave_loads": wave_loads, "seconds": best, "wave_loads_per_s": wave_loads / best,
+            "bytes_per_s": wave_loads * 1024 / best}
 
 
 def advect(field: DeviceField, positions: torch.Tensor, *, method="rk4", time_mode="frozen",

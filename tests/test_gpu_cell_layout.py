@@ -202,3 +202,51 @@ def test_cell_layouts_field_sample_bit_identical(pts_dtype):
     cell = _field(vels, pts, (n, n, n), N.LAYOUT_CELL, periodic=per)
     q = torch.as_tensor(rng.uniform(-1.0, 7.5, size=(700, 3)), device="cuda", dtype=pts_dtype)
     assert torch.equal(node.sample(q, time_mode=N.TIME_STEADY), cell.sample(q, time_mode=N.TIME_STEADY))
+
+
+@pytest.mark.parametrize("case", ["3d-steady-periodic", "2d-linear-pair"])
+def test_counting_launch_identical_and_bounded(case):
+    """The counting copy of the kernel (pyftle_advect_ftle_count_loads) gives the same
+    positions and FTLE bit for bit, and its count of wave-level record loads lies between
+    one record per wave (every lane's first sample loads) and every sample of every wave
+    loading (4 RK4 stages per step, 6 (3D) or 4 (2D pair) 16-byte loads per record)."""
+    from pyftle_amd import _native as N
+    from pyftle_amd import window as W
+
+    if case.startswith("3d"):
+        vels, pts = H.abc_window(n=20)
+        f = _field(vels, pts, (20, 20, 20), N.LAYOUT_CELL, periodic=(True, True, True))
+        x0 = H.seeds_3d(9)
+        kw = dict(method="rk4", time_mode="steady", n_steps=50, h=0.05, map_period=2.5)
+        nb, per_record = 6, 6
+    else:
+        vels, pts = H.double_gyre_window(nx=64, ny=32, n_snap=11)
+        f = _field(vels, pts, (64, 32), N.LAYOUT_CELL_PAIR)
+        x0 = _seeds_2d_with_outside()
+        kw = dict(method="rk4", time_mode="linear", n_steps=18, h=0.05, t_start=0.05, map_period=0.9)
+        nb, per_record = 4, 4
+    cfg = W.LaunchConfig(groups_per_block=32)
+    f0, p0 = _run(f, x0, cfg=cfg, **kw)
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    pos = torch.as_tensor(x0, device="cuda")
+    out = torch.empty_like(pos)
+    f1 = W.advect_ftle(f, pos, positions_out=out, cfg=cfg, count_loads=cnt, **kw)
+    assert torch.equal(f1, f0) and torch.equal(out, p0)
+    n_groups = x0.shape[0] // nb
+    waves = -(-n_groups // 32) * (-(-(32 * nb) // 64))
+    loads = int(cnt.item())
+    print(f"[count {case}] {loads} wave loads, {waves} waves, {kw['n_steps']} steps", flush=True)
+    assert waves * per_record <= loads <= waves * kw["n_steps"] * 4 * per_record
+    with pytest.raises(RuntimeError):  # node records are not counted
+        node = _field(vels, pts, (20, 20, 20) if case.startswith("3d") else (64, 32),
+                      N.LAYOUT_SINGLE if case.startswith("3d") else N.LAYOUT_PAIR,
+                      periodic=(True, True, True) if case.startswith("3d") else None)
+        W.advect_ftle(node, pos, cfg=cfg, count_loads=cnt, **kw)
+
+
+def test_probe_load_rate_is_positive():
+    from pyftle_amd import window as W
+
+    r = W.probe_load_rate(blocks=512, iters=1024, repeats=2)
+    print(f"[probe] {r['wave_loads_per_s']:.3e} wave loads/s = {r['bytes_per_s'] / 1e12:.1f} TB/s", flush=True)
+    assert r["wave_loads_per_s"] > 0
