@@ -692,7 +692,51 @@ def run_surrogate3d_infer(torch, wl, args):
     print(json.dumps(result), flush=True)
 
 
+def build_mlp_train(torch, batch=16384, depth=5, width=256, n_rows=1 << 21):
+    """One surrogate member (the 2D network: 64 Fourier frequencies on (x, y, t0), depth x
+    width SiLU, 2 outputs) trained on synthetic smooth targets, to time a training step."""
+    from pyftle_amd import surrogate as S
+
+    g = torch.Generator(device="cuda").manual_seed(0)
+    X = torch.rand(n_rows, 3, generator=g, device="cuda") * torch.tensor([2.0, 1.0, 10.0], device="cuda")
+    D = torch.stack([0.1 * torch.sin(3.0 * X[:, 0] + X[:, 2]) * torch.cos(2.0 * X[:, 1]),
+                     0.1 * torch.cos(2.0 * X[:, 0]) * torch.sin(3.0 * X[:, 1] - 0.5 * X[:, 2])], dim=1)
+    spec = S.MLPSpec(sigma=1.0, depth=depth, width=width)
+    return dict(name="mlp_train", X=X, D=D.contiguous(), spec=spec, batch=batch, S=S,
+                config={"workload": "mlp_train", "mlp": f"64 Fourier freqs on (x,y,t0), {depth}x{width} SiLU -> 2",
+                        "batch": batch, "rows": n_rows, "data": "synthetic smooth targets"})
+
+
+def run_mlp_train(torch, wl, args):
+    """Training steps of one member: forward, backward and Adam per minibatch, timed with
+    HIP events over whole epochs; flops = forward + input gradients + weight gradients."""
+    S = wl["S"]
+    spec, batch = wl["spec"], wl["batch"]
+    m = S.FlowMapMLP(spec, seed=1)
+    epochs = max(2, args.steps)
+    S.train(m, wl["X"], wl["D"], epochs=1, batch=batch, lr=3e-3, graph=args.train_graph)  # warm-up epoch
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    losses = S.train(m, wl["X"], wl["D"], epochs=epochs, batch=batch, lr=3e-3, graph=args.train_graph)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b)
+    steps = epochs * (wl["X"].shape[0] // batch)
+    flops = S.mlp_train_flops_per_row(spec) * batch * steps
+    tf = flops / (ms * 1e-3) / 1e12
+    log(f"{steps} steps of {batch} rows: {ms / steps:.3f} ms/step, {tf:.1f} TFLOP/s, loss {losses[-1]:.3e}")
+    print(json.dumps({"metric": "surrogate training TFLOP/s", "value": tf, "unit": "TFLOP/s", "n_gpus": 1,
+                      "steps": steps, "ms_per_step": ms / steps, "higher_is_better": True, "dtype": "bf16",
+                      "data": "synthetic", "config": dict(wl["config"], graph=bool(args.train_graph)),
+                      "mfma": {"achieved": tf, "peak": 2500.0, "unit": "TFLOP/s", "frac": tf / 2500.0,
+                               "flops_per_row": S.mlp_train_flops_per_row(spec)},
+                      "final_loss": losses[-1]}), flush=True)
+
+
 WORKLOADS = {"abc128": build_abc128, "gyre512": build_gyre, "batch64": build_batch64,
+             "mlp_train": build_mlp_train,
              "surrogate2d": build_surrogate2d, "surrogate_infer": build_surrogate_infer,
              "surrogate3d_infer": build_surrogate3d_infer,
              "surrogate3d": build_surrogate3d}
@@ -997,6 +1041,8 @@ def main():
     ap.add_argument("--surrogate-depth", type=int, default=0,
                     help="surrogate2d: hidden layers of width 256 (0 = the 5 of the specified network; "
                          "other depths are an accuracy/speed exploration)")
+    ap.add_argument("--train-batch", type=int, default=0, help="mlp_train: rows per step (0 = 16384)")
+    ap.add_argument("--train-graph", type=int, default=0, help="mlp_train: replay each step as a HIP graph")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy check")
     ap.add_argument("--seed-order", default="auto", choices=["auto", "morton", "none"],
@@ -1042,13 +1088,17 @@ def main():
         kw["epochs"] = args.surrogate_epochs
     if args.workload == "surrogate2d" and args.surrogate_depth > 0:
         kw["depth"] = args.surrogate_depth
+    if args.workload == "mlp_train" and args.train_batch > 0:
+        kw["batch"] = args.train_batch
     global RECORDS
     RECORDS = args.records
     wl = WORKLOADS[args.workload](torch, **kw)
-    if "config" in wl and args.workload not in ("surrogate3d", "surrogate3d_infer"):
+    if "config" in wl and args.workload not in ("surrogate3d", "surrogate3d_infer", "mlp_train"):
         wl["config"]["records"] = _records_name(wl.get("dim", 2))
     if wl["name"] == "batch64":
         return run_batch(torch, wl, args)
+    if wl["name"] == "mlp_train":
+        return run_mlp_train(torch, wl, args)
     if wl["name"] == "surrogate2d":
         return run_surrogate(torch, wl, args)
     if wl["name"] == "surrogate_infer":

@@ -428,6 +428,38 @@ int pyftle_mlp_fused_forward(const float *x, int32_t d_in, int32_t M, const floa
                              const float *bout, int32_t O, float out_scale, int32_t n_tangents,
                              float *Y, void *stream);
 
+/* Fused training step (one minibatch; BF16 MFMA, FP32 accumulation).  Rows
+ * *base_row .. *base_row + M - 1 of x (rows, d_in) and targets (rows, d_out; the targets
+ * of the network output, i.e. already divided by the output scale).  params is the flat
+ * FP32 layout of FlowMapMLP (per hidden layer W_l (width x fan_l) then b_l, then Wout
+ * (d_out x width), bout); W_packed holds the same weights in bf16 as written by
+ * pyftle_mlp_pack_train: the hidden layers in pyftle_mlp_pack_fused order followed by the
+ * transposed layers 1..depth-1.  Writes every parameter's MSE gradient (mean over the
+ * M x d_out outputs) to grads, adds the loss to *loss and advances *base_row by M, all on
+ * the stream, so a step has no host-side arguments that change.  M a multiple of 64 and
+ * of 64 * splits (split-K of the weight-gradient GEMMs); 64 Fourier frequencies, width
+ * 256 or 512, depth 2..8, d_in <= 4, d_out <= 3; work of pyftle_mlp_train_work_bytes,
+ * 256-byte aligned.  Gradients are summed in a fixed order (deterministic). */
+typedef struct pyftle_mlp_train_args {
+  const float *x;
+  const float *targets;
+  int64_t *base_row;      /* device */
+  int32_t M;
+  int32_t d_in, d_out, n_freq, width, depth;
+  int32_t splits;
+  const float *scale;     /* (d_in) */
+  const float *Bf;        /* (n_freq, d_in) */
+  const void *W_packed;
+  const float *params;
+  float *grads;
+  float *loss;            /* device FP32 */
+  void *work;
+} pyftle_mlp_train_args;
+int64_t pyftle_mlp_train_work_bytes(int32_t M, int32_t width, int32_t depth, int32_t d_out, int32_t splits);
+int pyftle_mlp_train_grads(const pyftle_mlp_train_args *args, void *stream);
+/* W_packed for pyftle_mlp_train_grads from the flat FP32 parameters. */
+int pyftle_mlp_pack_train(const float *params, int32_t width, int32_t depth, void *packed, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

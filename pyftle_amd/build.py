@@ -23,7 +23,7 @@ LIB_NAME = "libpyftle_hip.so"
 ARCH = os.environ.get("PYFTLE_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["capi.hip", "interp.hip", "spline.hip", "advect.hip", "advect_2d.hip", "advect_3d.hip",
-           "advect_rgi.hip", "ftle.hip", "stage.hip", "probe.hip", "batch.hip", "mlp.hip", "mlp_fused.hip"]
+           "advect_rgi.hip", "ftle.hip", "stage.hip", "probe.hip", "batch.hip", "mlp.hip", "mlp_fused.hip", "mlp_train.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

@@ -1,0 +1,616 @@
+// Fused surrogate training step: one minibatch of the Fourier-feature MLP in three
+// stages instead of ~45 layer-by-layer launches.
+//
+// No reference counterpart: the learned flow-map approximation of the RK4 path
+// (integrate.py:265-342) is this engine's addition.
+//
+// 1. train_kernel (one work-group per tile of 16*RT rows, activations in LDS as in the
+//    fused inference kernel): Fourier features, every hidden layer forward, the output
+//    layer and the MSE gradient, then the backward pass through the hidden layers with the
+//    transposed weights (dH_{l-1} = dZ_l W_l, the same MFMA loop with W_l^T packed), each
+//    dZ_l = dH_l * SiLU'(z_l).  It writes what the weight gradients need, transposed so that
+//    the batch rows are contiguous (the K dimension of the next stage): the features and
+//    hidden activations (HT) and every dZ (DZT); per-tile partial sums of the bias, output
+//    weight and output bias gradients and of the loss; and SiLU'(z) of the forward pass,
+//    in its own register order, for its own backward pass.
+// 2. Weight gradients dW_l = DZT_l HT_{l-1}^T: the BF16 MFMA GEMM of mlp.hip with both
+//    operands K-contiguous, split over K (FP32 partials).
+// 3. train_reduce_kernel: every parameter's gradient summed over splits or tiles in a fixed
+//    order (deterministic), the loss accumulated, the batch offset advanced.
+// Adam (pyftle_mlp_adam_scheduled) and pyftle_mlp_pack_train (forward and transposed
+// fragment-packed bf16 weights for the next step) complete a step.
+//
+// Transposed copies leave LDS as 16-byte stores: a lane takes a feature pair and 8 rows,
+// reading the 8 rows in an order rotated by its row octet so that the 64 lanes of a read
+// hit distinct banks, and rotates the words back in registers.
+
+#include "mlp_fused.hpp"
+
+namespace pyftle {
+namespace {
+
+using namespace mlpf;
+
+struct TrainArgs {
+  const float *x;          // (rows, d_in)
+  const float *t;          // (rows, O) targets of the network output (already / out_scale)
+  const int64_t *base_row; // first row of this step
+  int M;                   // rows per step
+  int d_in, depth;
+  const float *scale, *Bf;
+  const __bf16 *Wp;        // forward layers, then transposed layers 1..depth-1 (packed)
+  int wp_bytes;
+  const float *params;     // flat FP32 parameters
+  float loss_scale;        // 1 / (M * O)
+  __bf16 *FT, *HT, *DZT, *DACT;
+  float *bias_part, *wout_part, *bout_part, *loss_part;
+};
+
+// flat parameter layout: per layer W_l (W x fan_l) then b_l (W); then Wout (O x W), bout (O)
+template <int W>
+__host__ __device__ __forceinline__ int64_t w_off(int l) {
+  return l == 0 ? 0 : (int64_t)W * kFeat + W + (int64_t)(l - 1) * ((int64_t)W * W + W);
+}
+template <int W>
+__host__ __device__ __forceinline__ int64_t b_off(int l) {
+  return w_off<W>(l) + (int64_t)W * (l == 0 ? kFeat : W);
+}
+// byte offset of transposed layer l (1 <= l < depth) in the packed buffer
+template <int W>
+__host__ __device__ __forceinline__ int tlayer_bytes(int l, int depth) {
+  return 2 * (W * kFeat + (depth - 1) * W * W) + 2 * (l - 1) * W * W;
+}
+
+// LDS rows [0, 16 RT) x features [0, nfeat) -> dst[f * ld + col0 + row] (bf16)
+template <int W, int RT>
+__device__ __forceinline__ void transpose_out(const __bf16 *act, int nfeat, __bf16 *dst, int64_t ld,
+                                              int64_t col0, int tid) {
+  constexpr int Q = 2 * RT;  // row octets
+  const int tasks = (nfeat / 2) * Q;
+  for (int task = tid; task < tasks; task += kThreads) {
+    const int q = task % Q, p = task / Q;
+    const int rot = (q >> 1) & 7;
+    const int f = 2 * p;
+    uint32_t v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int r = 8 * q + ((j + rot) & 7);
+      v[j] = *reinterpret_cast<const uint32_t *>(act + act_off<W>(r, f >> 3) + (f & 7));
+    }
+    // v[j] holds row 8 q + ((j + rot) & 7); rotate so that v[k] holds row 8 q + k
+#pragma unroll
+    for (int s = 1; s < 8; s <<= 1) {
+      uint32_t w[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) w[k] = (rot & s) ? v[(k - s) & 7] : v[k];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = w[k];
+    }
+    uint4 lo, hi;  // feature f (low halves), feature f + 1 (high halves), rows 8 q .. 8 q + 7
+    lo.x = (v[0] & 0xffffu) | (v[1] << 16);
+    lo.y = (v[2] & 0xffffu) | (v[3] << 16);
+    lo.z = (v[4] & 0xffffu) | (v[5] << 16);
+    lo.w = (v[6] & 0xffffu) | (v[7] << 16);
+    hi.x = (v[0] >> 16) | (v[1] & 0xffff0000u);
+    hi.y = (v[2] >> 16) | (v[3] & 0xffff0000u);
+    hi.z = (v[4] >> 16) | (v[5] & 0xffff0000u);
+    hi.w = (v[6] >> 16) | (v[7] & 0xffff0000u);
+    *reinterpret_cast<uint4 *>(dst + (int64_t)f * ld + col0 + 8 * q) = lo;
+    *reinterpret_cast<uint4 *>(dst + (int64_t)(f + 1) * ld + col0 + 8 * q) = hi;
+  }
+}
+
+// Per-feature sum over this lane's rows (its RT row tiles), then over the 16 lanes that
+// share the lane's features (lane % 16 = row within a tile): lanes with lane % 16 == 0 write.
+template <int W, int RT>
+__device__ __forceinline__ void feature_sums(const f32x4_t (&acc)[Shape<W, RT>::kTilesPerWave][RT], float *dst,
+                                             int ft0, int lane_in) {
+  const int lane = opaque_lane(lane_in);
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < Shape<W, RT>::kTilesPerWave; ++i) {
+    float s[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      float v = 0.0f;
+#pragma unroll
+      for (int t = 0; t < RT; ++t) v += acc[i][t][e];
+      v += __shfl_xor(v, 1);
+      v += __shfl_xor(v, 2);
+      v += __shfl_xor(v, 4);
+      v += __shfl_xor(v, 8);
+      s[e] = v;
+    }
+    if (r == 0) *reinterpret_cast<f32x4_t *>(dst + (ft0 + i) * 16 + 4 * g) = f32x4_t{s[0], s[1], s[2], s[3]};
+  }
+}
+
+// acc (FP32) as bf16 activation rows of LDS (the layout the MFMA loop reads)
+template <int W, int RT>
+__device__ __forceinline__ void store_rows(const f32x4_t (&acc)[Shape<W, RT>::kTilesPerWave][RT], __bf16 *act,
+                                           int ft0, int lane_in) {
+  const int lane = opaque_lane(lane_in);
+  const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < Shape<W, RT>::kTilesPerWave; ++i) {
+    const int f0 = (ft0 + i) * 16 + 4 * g;
+    const int off = act_off<W>(r, f0 >> 3) + (f0 & 7);
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+      bf16x4_t h;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) h[e] = (__bf16)acc[i][t][e];
+      *reinterpret_cast<bf16x4_t *>(act + off + t * 16 * W) = h;
+    }
+  }
+}
+
+// Two work-groups per CU at width 256 (one's epilogues under the other's MFMAs); at width
+// 512 the accumulators alone take half the registers of two, so one.
+template <int W, int RT, int O>
+__global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(TrainArgs a) {
+  using Sh = Shape<W, RT>;
+  constexpr int TPW = Sh::kTilesPerWave, kRows = Sh::kRows;
+  __shared__ __attribute__((aligned(16))) __bf16 act[kRows * W];
+  __shared__ __attribute__((aligned(16))) float freq_s[kNFreq * kMaxIn];
+  __shared__ float red[kWaves][kRows][4];
+  __shared__ float dy_s[kRows][4];
+  __shared__ float sq_s[kRows * 4];
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ft0 = wave * TPW;
+  const int tile = blockIdx.x, ntiles = gridDim.x;
+  const int64_t grow0 = *a.base_row + (int64_t)tile * kRows;  // data row of the tile's row 0
+  const int64_t col0 = (int64_t)tile * kRows;                 // its column in the step's buffers
+  const int64_t M = a.M;
+  {
+    const int q = tid >> 2, c = tid & 3;
+    float fv = 0.0f, sc = 0.0f;
+    if (c < a.d_in) {
+      fv = a.Bf[q * a.d_in + c];
+      sc = a.scale[c];
+    }
+    freq_s[tid] = fv * sc;
+  }
+  const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void *)a.Wp, 0, a.wp_bytes, 0x00020000);
+  bf16x8_t ring[TPW];
+#pragma unroll
+  for (int i = 0; i < TPW; ++i) ring[i] = load_frag(wr, lane * 16, frag_bytes<W>(0, ft0 + i));
+  __syncthreads();
+
+  // Fourier features [sin | cos] of the tile's rows
+  for (int task = tid; task < kRows * 8; task += kThreads) {
+    const int j = task >> 3, fb = task & 7;
+    float xin[kMaxIn];
+#pragma unroll
+    for (int c = 0; c < kMaxIn; ++c) xin[c] = c < a.d_in ? a.x[(grow0 + j) * a.d_in + c] : 0.0f;
+    bf16x8_t sn, cs;
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) {
+      const f32x4_t fr = *reinterpret_cast<const f32x4_t *>(freq_s + (fb * 8 + jj) * kMaxIn);
+      float ph = 0.0f;
+#pragma unroll
+      for (int c = 0; c < kMaxIn; ++c) ph += fr[c] * xin[c];
+      ph -= rintf(ph);
+      sn[jj] = (__bf16)__builtin_amdgcn_sinf(ph);
+      cs[jj] = (__bf16)__builtin_amdgcn_cosf(ph);
+    }
+    *reinterpret_cast<bf16x8_t *>(act + act_off<W>(j, fb)) = sn;
+    *reinterpret_cast<bf16x8_t *>(act + act_off<W>(j, kNFreq / 8 + fb)) = cs;
+  }
+  __syncthreads();
+  transpose_out<W, RT>(act, kFeat, a.FT, M, col0, tid);
+
+  f32x4_t acc[TPW][RT];
+  // Forward.  Layer 0 (K = 128) and the last layer are peeled off the loop so that no
+  // accumulator array is carried around a loop holding two instantiations of the MFMA loop.
+  layer_mma<W, RT, kFeat>(wr, layer_bytes<W>(0), layer_bytes<W>(1), act, ring, acc, ft0, lane);
+  for (int L = 0;; ++L) {
+    __syncthreads();  // every wave has read this layer's inputs (and their transposed copy is out)
+    {
+      const float *bl = a.params + b_off<W>(L);
+      const int ln = opaque_lane(lane);
+      const int r = ln & 15, g = ln >> 4;
+      __bf16 *dact = a.DACT + ((((int64_t)L * ntiles + tile) * kWaves + wave) * TPW * RT * 64) * 4;
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) {
+        const int f0 = (ft0 + i) * 16 + 4 * g;
+        const f32x4_t bv = *reinterpret_cast<const f32x4_t *>(bl + f0);
+        const int off = act_off<W>(r, f0 >> 3) + (f0 & 7);
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+          bf16x4_t h, d;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const float z = acc[i][t][e] + bv[e];
+            const float s = sigmoid_f(z);
+            h[e] = (__bf16)(z * s);
+            d[e] = (__bf16)(s * (1.0f + z * (1.0f - s)));
+          }
+          *reinterpret_cast<bf16x4_t *>(act + off + t * 16 * W) = h;
+          *reinterpret_cast<bf16x4_t *>(dact + ((i * RT + t) * 64 + ln) * 4) = d;
+        }
+      }
+    }
+    __syncthreads();
+    transpose_out<W, RT>(act, W, a.HT + (int64_t)L * W * M, M, col0, tid);
+    if (L + 2 == a.depth) break;
+    layer_mma<W, RT, W>(wr, layer_bytes<W>(L + 1), layer_bytes<W>(L + 2), act, ring, acc, ft0, lane);
+  }
+  {
+    const int L = a.depth - 1;
+    layer_mma<W, RT, W>(wr, layer_bytes<W>(L), tlayer_bytes<W>(L, a.depth), act, ring, acc, ft0, lane);
+    const float *bl = a.params + b_off<W>(L);
+    __syncthreads();  // every wave has read the last layer's inputs
+    // output layer (FP32): y = SiLU(z + b) Wout^T + bout, then the MSE gradient
+    const float *wout = a.params + w_off<W>(a.depth);
+    const float *bout = wout + O * W;
+    {
+      const int g = opaque_lane(lane) >> 4;
+      float yp[RT][O];
+#pragma unroll
+      for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int o = 0; o < O; ++o) yp[t][o] = 0.0f;
+#pragma unroll
+      for (int i = 0; i < TPW; ++i) {
+        const int f0 = (ft0 + i) * 16 + 4 * g;
+        const f32x4_t bv = *reinterpret_cast<const f32x4_t *>(bl + f0);
+        f32x4_t wv[O];
+#pragma unroll
+        for (int o = 0; o < O; ++o) wv[o] = *reinterpret_cast<const f32x4_t *>(wout + o * W + f0);
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const float z = acc[i][t][e] + bv[e];
+            const float hz = z * sigmoid_f(z);
+#pragma unroll
+            for (int o = 0; o < O; ++o) yp[t][o] = fmaf(hz, wv[o][e], yp[t][o]);
+          }
+      }
+#pragma unroll
+      for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int o = 0; o < O; ++o) {
+          float v = yp[t][o];
+          v += __shfl_xor(v, 16);
+          v += __shfl_xor(v, 32);
+          yp[t][o] = v;
+        }
+      if (lane < 16) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+          for (int o = 0; o < O; ++o) red[wave][t * 16 + lane][o] = yp[t][o];
+      }
+    }
+    __syncthreads();
+    if (tid < kRows * O) {
+      const int n = tid / O, o = tid % O;
+      float y = bout[o];
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) y += red[w][n][o];
+      const float diff = y - a.t[(grow0 + n) * O + o];
+      dy_s[n][o] = 2.0f * diff * a.loss_scale;
+      sq_s[tid] = diff * diff * a.loss_scale;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      float l = 0.0f;
+      for (int k = 0; k < kRows * O; ++k) l += sq_s[k];
+      a.loss_part[tile] = l;
+    } else if (tid <= O) {
+      float sb = 0.0f;
+      for (int n = 0; n < kRows; ++n) sb += dy_s[n][tid - 1];
+      a.bout_part[tile * 4 + tid - 1] = sb;
+    }
+    // dZ of the last hidden layer and the output weights' gradient partials
+    float *wpart = a.wout_part + (int64_t)tile * O * W;
+    const int ln = opaque_lane(lane);
+    const int r = ln & 15, g = ln >> 4;
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) {
+      const int f0 = (ft0 + i) * 16 + 4 * g;
+      const f32x4_t bv = *reinterpret_cast<const f32x4_t *>(bl + f0);
+      f32x4_t wv[O];
+#pragma unroll
+      for (int o = 0; o < O; ++o) wv[o] = *reinterpret_cast<const f32x4_t *>(wout + o * W + f0);
+      float ws[O][4];
+#pragma unroll
+      for (int o = 0; o < O; ++o)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ws[o][e] = 0.0f;
+#pragma unroll
+      for (int t = 0; t < RT; ++t) {
+        float dyv[O];
+#pragma unroll
+        for (int o = 0; o < O; ++o) dyv[o] = dy_s[t * 16 + r][o];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float z = acc[i][t][e] + bv[e];
+          const float s = sigmoid_f(z);
+          const float hz = z * s;
+          float dh = 0.0f;
+#pragma unroll
+          for (int o = 0; o < O; ++o) {
+            dh = fmaf(dyv[o], wv[o][e], dh);
+            ws[o][e] = fmaf(dyv[o], hz, ws[o][e]);
+          }
+          acc[i][t][e] = dh * (s * (1.0f + z * (1.0f - s)));
+        }
+      }
+#pragma unroll
+      for (int o = 0; o < O; ++o)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float v = ws[o][e];
+          v += __shfl_xor(v, 1);
+          v += __shfl_xor(v, 2);
+          v += __shfl_xor(v, 4);
+          v += __shfl_xor(v, 8);
+          ws[o][e] = v;
+        }
+      if (r == 0) {
+#pragma unroll
+        for (int o = 0; o < O; ++o)
+          *reinterpret_cast<f32x4_t *>(wpart + o * W + f0) = f32x4_t{ws[o][0], ws[o][1], ws[o][2], ws[o][3]};
+      }
+    }
+  }
+  // backward through the hidden layers: acc holds dZ of layer l
+  for (int l = a.depth - 1;; --l) {
+    feature_sums<W, RT>(acc, a.bias_part + ((int64_t)tile * a.depth + l) * W, ft0, lane);
+    store_rows<W, RT>(acc, act, ft0, lane);  // every wave finished reading act before the last barrier
+    __syncthreads();
+    transpose_out<W, RT>(act, W, a.DZT + (int64_t)l * W * M, M, col0, tid);
+    if (l == 0) break;
+    const int lb = tlayer_bytes<W>(l, a.depth);
+    const int nlb = l > 1 ? tlayer_bytes<W>(l - 1, a.depth) : layer_bytes<W>(0);
+    layer_mma<W, RT, W>(wr, lb, nlb, act, ring, acc, ft0, lane);  // dH of layer l - 1
+    __syncthreads();
+    const __bf16 *dact = a.DACT + (((((int64_t)(l - 1) * ntiles + tile) * kWaves + wave) * TPW * RT * 64) * 4);
+    const int ln = opaque_lane(lane);
+#pragma unroll
+    for (int i = 0; i < TPW; ++i)
+#pragma unroll
+      for (int t = 0; t < RT; ++t) {
+        const bf16x4_t d = *reinterpret_cast<const bf16x4_t *>(dact + ((i * RT + t) * 64 + ln) * 4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[i][t][e] *= (float)d[e];
+      }
+  }
+}
+
+struct ReduceArgs {
+  const float *part;       // weight-gradient partials, layer after layer: (splits, W, fan_l)
+  int splits;
+  const float *bias_part, *wout_part, *bout_part, *loss_part;
+  int tiles, depth, W, O;
+  int64_t n_params;
+  float *grads;
+  float *loss;             // += the step's loss
+  int64_t *base_row;       // += M
+  int M;
+};
+
+__global__ void __launch_bounds__(256) train_reduce_kernel(ReduceArgs a) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) {
+    float l = 0.0f;
+    for (int k = 0; k < a.tiles; ++k) l += a.loss_part[k];
+    *a.loss += l;
+    *a.base_row += a.M;
+  }
+  if (p >= a.n_params) return;
+  const int W = a.W;
+  int64_t off = 0, poff = 0;
+  for (int l = 0; l < a.depth; ++l) {
+    const int fan = l == 0 ? kFeat : W;
+    const int64_t nw = (int64_t)W * fan;
+    if (p < off + nw) {  // weight (o, i) of layer l
+      const int64_t loc = p - off;
+      const float *src = a.part + poff + loc;
+      float gsum = 0.0f;
+      for (int s = 0; s < a.splits; ++s) gsum += src[(int64_t)s * nw];
+      a.grads[p] = gsum;
+      return;
+    }
+    if (p < off + nw + W) {  // bias o of layer l
+      const int o = (int)(p - off - nw);
+      float gsum = 0.0f;
+      for (int k = 0; k < a.tiles; ++k) gsum += a.bias_part[((int64_t)k * a.depth + l) * W + o];
+      a.grads[p] = gsum;
+      return;
+    }
+    off += nw + W;
+    poff += (int64_t)a.splits * nw;
+  }
+  const int64_t loc = p - off;
+  float gsum = 0.0f;
+  if (loc < (int64_t)a.O * W) {  // output weight (o, f)
+    for (int k = 0; k < a.tiles; ++k) gsum += a.wout_part[(int64_t)k * a.O * W + loc];
+  } else {  // output bias
+    const int o = (int)(loc - (int64_t)a.O * W);
+    for (int k = 0; k < a.tiles; ++k) gsum += a.bout_part[k * 4 + o];
+  }
+  a.grads[p] = gsum;
+}
+
+// index of element (m, k) of a rows x K matrix in the fragment-packed order of
+// pyftle_mlp_pack_fused
+__device__ __forceinline__ int64_t packed_index(int m, int k, int rows) {
+  const int ft = m >> 4, ks = k >> 5, ln = (m & 15) + 16 * ((k & 31) >> 3);
+  return (((int64_t)ks * (rows >> 4) + ft) * 64 + ln) * 8 + (k & 7);
+}
+
+__global__ void __launch_bounds__(256) pack_train_kernel(const float *params, int W, int depth, __bf16 *out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t off = 0, pk = 0;  // parameter offset and packed forward offset (elements) of layer l
+  for (int l = 0; l < depth; ++l) {
+    const int fan = l == 0 ? kFeat : W;
+    const int64_t nw = (int64_t)W * fan;
+    if (q < pk + nw) {
+      const int64_t loc = q - pk;
+      const int o = (int)(loc / fan), i = (int)(loc % fan);
+      const __bf16 v = (__bf16)params[off + loc];
+      out[pk + packed_index(o, i, W)] = v;
+      if (l > 0) {  // W_l^T: fan x W, element (i, o)
+        const int64_t tb = (int64_t)W * kFeat + (int64_t)(depth - 1) * W * W + (int64_t)(l - 1) * W * W;
+        out[tb + packed_index(i, o, fan)] = v;
+      }
+      return;
+    }
+    off += nw + W;
+    pk += nw;
+  }
+}
+
+template <int W, int RT, int O>
+int launch_train(const TrainArgs &a, hipStream_t st) {
+  train_kernel<W, RT, O><<<a.M / (16 * RT), kThreads, 0, st>>>(a);
+  return check_launch("mlp_train_grads");
+}
+
+int64_t align256(int64_t n) { return (n + 255) / 256 * 256; }
+
+struct WorkLayout {
+  int64_t ft, ht, dzt, dact, part, bias, wout, bout, loss, total;
+};
+
+WorkLayout work_layout(int64_t M, int W, int depth, int O, int splits, int rows_per_tile) {
+  WorkLayout w{};
+  const int64_t tiles = M / rows_per_tile;
+  int64_t nw = 0;
+  for (int l = 0; l < depth; ++l) nw += (int64_t)W * (l == 0 ? kFeat : W);
+  int64_t o = 0;
+  w.ft = o; o += align256(2 * kFeat * M);
+  w.ht = o; o += align256(2 * (int64_t)(depth - 1) * W * M);
+  w.dzt = o; o += align256(2 * (int64_t)depth * W * M);
+  w.dact = o; o += align256(2 * (int64_t)(depth - 1) * W * M);
+  w.part = o; o += align256(4 * (int64_t)splits * nw);
+  w.bias = o; o += align256(4 * tiles * depth * W);
+  w.wout = o; o += align256(4 * tiles * O * W);
+  w.bout = o; o += align256(4 * tiles * 4);
+  w.loss = o; o += align256(4 * tiles);
+  w.total = o;
+  return w;
+}
+
+// rows per work-group: 64 (4 row tiles of 16) at both widths, so a 16384-row step spreads
+// over 256 work-groups
+int tile_rows(int) { return 64; }
+
+// forward layers then transposed layers 1..depth-1, bf16
+int packed_train_bytes(int W, int depth) { return 2 * (W * kFeat + 2 * (depth - 1) * W * W); }
+
+}  // namespace
+}  // namespace pyftle
+
+using namespace pyftle;
+
+extern "C" {
+
+int64_t pyftle_mlp_train_work_bytes(int32_t M, int32_t width, int32_t depth, int32_t O, int32_t splits) {
+  if (M <= 0 || (width != 256 && width != 512) || depth < 2 || depth > mlpf::kMaxDepth || O < 1 || O > 3 ||
+      splits < 1)
+    return -1;
+  return work_layout(M, width, depth, O, splits, tile_rows(width)).total;
+}
+
+int pyftle_mlp_train_grads(const pyftle_mlp_train_args *t, void *stream) {
+  if (!t) return fail("mlp_train_grads: NULL arguments");
+  if (!t->x || !t->targets || !t->base_row || !t->scale || !t->Bf || !t->W_packed || !t->params || !t->grads ||
+      !t->loss || !t->work)
+    return fail("mlp_train_grads: NULL pointer");
+  if (t->n_freq != mlpf::kNFreq || (t->width != 256 && t->width != 512))
+    return fail("mlp_train_grads: needs 64 Fourier frequencies and hidden width 256 or 512");
+  if (t->depth < 2 || t->depth > mlpf::kMaxDepth) return fail("mlp_train_grads: depth must be in [2, 8]");
+  if (t->d_in < 1 || t->d_in > mlpf::kMaxIn || t->d_out < 1 || t->d_out > 3)
+    return fail("mlp_train_grads: 1 <= d_in <= 4 and 1 <= d_out <= 3");
+  const int R = tile_rows(t->width);
+  const int S = t->splits;
+  if (t->M <= 0 || t->M % R || S < 1 || t->M % (64 * S))
+    return fail("mlp_train_grads: M must be a multiple of " + std::to_string(R) + " and of 64 * splits");
+  if ((uintptr_t)t->params % 16 || (uintptr_t)t->W_packed % 16 || (uintptr_t)t->work % 256)
+    return fail("mlp_train_grads: params and W_packed 16-byte, work 256-byte aligned");
+  const int W = t->width, depth = t->depth, O = t->d_out;
+  const WorkLayout wl = work_layout(t->M, W, depth, O, S, R);
+  char *wk = static_cast<char *>(t->work);
+  TrainArgs a{};
+  a.x = t->x;
+  a.t = t->targets;
+  a.base_row = t->base_row;
+  a.M = t->M;
+  a.d_in = t->d_in;
+  a.depth = depth;
+  a.scale = t->scale;
+  a.Bf = t->Bf;
+  a.Wp = static_cast<const __bf16 *>(t->W_packed);
+  a.wp_bytes = packed_train_bytes(W, depth);
+  a.params = t->params;
+  a.loss_scale = 1.0f / ((float)t->M * (float)O);
+  a.FT = reinterpret_cast<__bf16 *>(wk + wl.ft);
+  a.HT = reinterpret_cast<__bf16 *>(wk + wl.ht);
+  a.DZT = reinterpret_cast<__bf16 *>(wk + wl.dzt);
+  a.DACT = reinterpret_cast<__bf16 *>(wk + wl.dact);
+  a.bias_part = reinterpret_cast<float *>(wk + wl.bias);
+  a.wout_part = reinterpret_cast<float *>(wk + wl.wout);
+  a.bout_part = reinterpret_cast<float *>(wk + wl.bout);
+  a.loss_part = reinterpret_cast<float *>(wk + wl.loss);
+  hipStream_t st = as_stream(stream);
+  int rc;
+  if (W == 256) {
+    rc = O == 1 ? launch_train<256, 4, 1>(a, st) : O == 2 ? launch_train<256, 4, 2>(a, st)
+                                                            : launch_train<256, 4, 3>(a, st);
+  } else {
+    rc = O == 1 ? launch_train<512, 4, 1>(a, st) : O == 2 ? launch_train<512, 4, 2>(a, st)
+                                                            : launch_train<512, 4, 3>(a, st);
+  }
+  if (rc) return rc;
+  // weight gradients: dW_l (W x fan) = DZT_l (W x M) HT_{l-1}^T, both K-contiguous, split over K
+  float *part = reinterpret_cast<float *>(wk + wl.part);
+  int64_t poff = 0;
+  for (int l = 0; l < depth; ++l) {
+    const int fan = l == 0 ? mlpf::kFeat : W;
+    const __bf16 *A = a.DZT + (int64_t)l * W * t->M;
+    const __bf16 *B = l == 0 ? a.FT : a.HT + (int64_t)(l - 1) * W * t->M;
+    rc = pyftle_mlp_gemm(A, t->M, 1, B, t->M, 1, W, fan, t->M, S, 0, nullptr, nullptr, 0, part + poff, fan,
+                         stream);
+    if (rc) return rc;
+    poff += (int64_t)S * W * fan;
+  }
+  ReduceArgs r{};
+  r.part = part;
+  r.splits = S;
+  r.bias_part = a.bias_part;
+  r.wout_part = a.wout_part;
+  r.bout_part = a.bout_part;
+  r.loss_part = a.loss_part;
+  r.tiles = t->M / R;
+  r.depth = depth;
+  r.W = W;
+  r.O = O;
+  int64_t n = 0;
+  for (int l = 0; l < depth; ++l) n += (int64_t)W * (l == 0 ? mlpf::kFeat : W) + W;
+  r.n_params = n + (int64_t)O * W + O;
+  r.grads = t->grads;
+  r.loss = t->loss;
+  r.base_row = t->base_row;
+  r.M = t->M;
+  train_reduce_kernel<<<(unsigned)((r.n_params + 255) / 256), 256, 0, st>>>(r);
+  return check_launch("mlp_train_grads reduce");
+}
+
+int pyftle_mlp_pack_train(const float *params, int32_t width, int32_t depth, void *packed, void *stream) {
+  if (!params || !packed) return fail("mlp_pack_train: NULL pointer");
+  if ((width != 256 && width != 512) || depth < 2 || depth > mlpf::kMaxDepth)
+    return fail("mlp_pack_train: width 256 or 512, depth in [2, 8]");
+  const int64_t n = (int64_t)width * mlpf::kFeat + (int64_t)(depth - 1) * width * width;
+  pack_train_kernel<<<(unsigned)((n + 255) / 256), 256, 0, as_stream(stream)>>>(params, width, depth,
+                                                                               static_cast<__bf16 *>(packed));
+  return check_launch("mlp_pack_train");
+}
+
+}  // extern "C"

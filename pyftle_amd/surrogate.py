@@ -45,7 +45,18 @@ _SIGS = {
                                   _vp, _vp],
     "pyftle_mlp_fused_forward": [_vp, _i32, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _f32,
                                  _i32, _vp, _vp],
+    "pyftle_mlp_train_work_bytes": [_i32, _i32, _i32, _i32, _i32],
+    "pyftle_mlp_train_grads": [_vp, _vp],
+    "pyftle_mlp_pack_train": [_vp, _i32, _i32, _vp, _vp],
 }
+
+
+class TrainArgs(ctypes.Structure):
+    """``pyftle_mlp_train_args`` (include/pyftle_hip.h)."""
+
+    _fields_ = [("x", _vp), ("targets", _vp), ("base_row", _vp), ("M", _i32), ("d_in", _i32), ("d_out", _i32),
+                ("n_freq", _i32), ("width", _i32), ("depth", _i32), ("splits", _i32), ("scale", _vp), ("Bf", _vp),
+                ("W_packed", _vp), ("params", _vp), ("grads", _vp), ("loss", _vp), ("work", _vp)]
 # (outputs, tangents) covered by the one-kernel network, per hidden width
 _FUSED_SHAPES = {256: ((2, 0), (2, 2), (3, 0)), 512: ((2, 0), (3, 0), (3, 3))}
 _registered = False
@@ -56,6 +67,7 @@ def _lib():
     global _registered
     if not _registered:
         N.extra_signatures(_SIGS)
+        N.load().pyftle_mlp_train_work_bytes.restype = ctypes.c_int64
         _registered = True
     return N.load()
 
@@ -347,17 +359,70 @@ class FlowMapMLP:
                _p(self.params_bf16), stream_handle())
 
 
+def fused_train_supported(spec: MLPSpec, batch: int) -> bool:
+    """Whether ``pyftle_mlp_train_grads`` covers this network and minibatch."""
+    return (spec.n_freq == 64 and spec.width in (256, 512) and 2 <= spec.depth <= 8 and spec.d_in <= 4
+            and spec.d_out <= 3 and batch % 64 == 0)
+
+
+class FusedTrainStep:
+    """One minibatch step on the fused training kernels: gradients of rows
+    ``*base .. *base + batch - 1`` of (X, T / out_scale) (``pyftle_mlp_train_grads``, which
+    also adds the loss to ``loss`` and advances ``base``), the scheduled Adam update and the
+    repacking of the bf16 weights the next step reads (``pyftle_mlp_pack_train``)."""
+
+    def __init__(self, model: FlowMapMLP, X: torch.Tensor, Tn: torch.Tensor, batch: int, base: torch.Tensor,
+                 loss: torch.Tensor, splits: int = 0):
+        _lib()
+        s = model.spec
+        if not fused_train_supported(s, batch):
+            raise ValueError(f"the fused training step does not cover {s} with batch {batch}")
+        if splits <= 0:
+            splits = max(1, min(16, batch // 1024))
+        while batch % (64 * splits):
+            splits -= 1
+        self.model, self.batch, self.splits = model, batch, splits
+        dev = X.device
+        nbytes = N.load().pyftle_mlp_train_work_bytes(batch, s.width, s.depth, s.d_out, splits)
+        if nbytes <= 0:
+            raise ValueError("unsupported fused training shape")
+        self.work = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
+        n_packed = s.width * 2 * s.n_freq + 2 * (s.depth - 1) * s.width * s.width
+        self.packed = torch.empty(n_packed, dtype=torch.bfloat16, device=dev)
+        self.repack()
+        a = TrainArgs()
+        a.x, a.targets, a.base_row = X.data_ptr(), Tn.data_ptr(), base.data_ptr()
+        a.M, a.d_in, a.d_out, a.n_freq = batch, s.d_in, s.d_out, s.n_freq
+        a.width, a.depth, a.splits = s.width, s.depth, splits
+        a.scale, a.Bf = model.scale.data_ptr(), model.B.data_ptr()
+        a.W_packed, a.params, a.grads = self.packed.data_ptr(), model.params.data_ptr(), model.grads.data_ptr()
+        a.loss, a.work = loss.data_ptr(), self.work.data_ptr()
+        self.args = a
+        self._keep = (X, Tn, base, loss)
+
+    def repack(self):
+        s = self.model.spec
+        N.call("pyftle_mlp_pack_train", _p(self.model.params), s.width, s.depth, _p(self.packed), stream_handle())
+
+    def grads(self):
+        N.call("pyftle_mlp_train_grads", ctypes.byref(self.args), stream_handle())
+
+
 def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20, batch: int = 32768,
-          lr: float = 2e-3, lr_min: float = 2e-5, seed: int = 0, log=None, graph: bool = False):
+          lr: float = 2e-3, lr_min: float = 2e-5, seed: int = 0, log=None, graph: bool = False,
+          fused: bool = True):
     """Adam with a cosine learning-rate schedule over minibatches; returns per-epoch losses.
 
     Each epoch gathers a shuffled copy of the data once; a step takes the next ``batch``
     rows at a device-side offset, runs forward, backward and the device-scheduled Adam
     update (``pyftle_mlp_adam_scheduled``), so no launch argument changes between steps.
+    ``fused`` (default, where ``fused_train_supported``): forward and backward in the fused
+    training kernels (``FusedTrainStep``: one kernel per row tile for the activations, the
+    split-K weight-gradient GEMMs, one reduction), about 10 launches a step; otherwise the
+    layer-by-layer kernels (~45 launches).
     With ``graph`` the step is captured in a HIP graph after two eager steps and replayed;
     eager and replayed steps run the same kernels on the same buffers (bit-identical
-    weights).  At width 256 and 16384 rows a step (~40 kernels, 0.56 ms) is GPU-bound, so
-    replay gains nothing there and eager is the default.
+    weights).
     """
     n = X.shape[0]
     if n < batch:
@@ -375,14 +440,30 @@ def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20,
     acc = torch.zeros(1, dtype=torch.float32, device=dev)
     step0 = model.step
 
+    use_fused = fused and fused_train_supported(model.spec, batch)
+    if use_fused:
+        # targets of the network output, shuffled with the inputs each epoch
+        Tn = (T / model.spec.out_scale).contiguous()
+        fstep = FusedTrainStep(model, Xs, Ts, batch, base, acc)
+    else:
+        Tn = T
+
+    def adam():
+        N.call("pyftle_mlp_adam_scheduled", _p(model.params), _p(model.m), _p(model.v), _p(model.grads),
+               model.n_params, 0.9, 0.999, 1e-8, 0.0, float(lr), float(lr_min), total, step0, _p(k_dev),
+               _p(model.params_bf16), stream_handle())
+
     def body():
+        if use_fused:
+            fstep.grads()  # also advances base
+            adam()
+            fstep.repack()
+            return
         idx = base + ar
         torch.index_select(Xs, 0, idx, out=xb)
         torch.index_select(Ts, 0, idx, out=tb)
         model.loss_and_backward(xb, tb, loss_out=acc)
-        N.call("pyftle_mlp_adam_scheduled", _p(model.params), _p(model.m), _p(model.v), _p(model.grads),
-               model.n_params, 0.9, 0.999, 1e-8, 0.0, float(lr), float(lr_min), total, step0, _p(k_dev),
-               _p(model.params_bf16), stream_handle())
+        adam()
         base.add_(batch)
 
     g = None
@@ -391,7 +472,7 @@ def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20,
     for ep in range(epochs):
         perm = torch.randperm(n, generator=gen, device=dev)
         torch.index_select(X, 0, perm, out=Xs)
-        torch.index_select(T, 0, perm, out=Ts)
+        torch.index_select(Tn, 0, perm, out=Ts)
         base.zero_()
         acc.zero_()
         for i in range(steps_per_epoch):
@@ -467,5 +548,13 @@ def mlp_flops_per_query(spec: MLPSpec) -> int:
     return f + 2 * spec.width * spec.d_out
 
 
+def mlp_train_flops_per_row(spec: MLPSpec) -> int:
+    """Flops of one training row: the forward pass, the input gradients of every layer but
+    the first (whose input, the Fourier features, needs none) and every weight gradient."""
+    fwd = mlp_flops_per_query(spec)
+    dgrad = fwd - 2 * 2 * spec.n_freq * spec.width
+    return fwd + dgrad + fwd
+
+
 __all__ = ["MLPSpec", "FlowMapMLP", "FlowMapEnsemble", "train", "ftle_from_displacement_jacobian",
-           "gemm", "pad128", "mlp_flops_per_query"]
+           "gemm", "pad128", "mlp_flops_per_query", "mlp_train_flops_per_row"]

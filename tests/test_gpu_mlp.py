@@ -293,3 +293,69 @@ def test_graph_captured_training_equals_eager():
     assert torch.equal(m_eager.params_bf16, m_graph.params_bf16)
     np.testing.assert_allclose(l_graph, l_eager, rtol=1e-5)
     assert m_graph.step == m_eager.step == 24
+
+
+def _train_model(width, depth, d_in, d_out, seed=7):
+    from pyftle_amd.surrogate import FlowMapMLP, MLPSpec
+
+    spec = MLPSpec(d_in=d_in, d_out=d_out, n_freq=64, width=width, depth=depth, sigma=1.0,
+                   scale=(0.5, 1.0, 0.1, 0.3)[:d_in], out_scale=0.7)
+    return FlowMapMLP(spec, seed=seed)
+
+
+@pytest.mark.parametrize("width,depth,d_in,d_out", [(256, 5, 3, 2), (256, 3, 3, 3), (512, 3, 4, 3)])
+def test_fused_training_step_gradients_match_oracle(width, depth, d_in, d_out):
+    """pyftle_mlp_train_grads: every parameter's MSE gradient of one minibatch against the
+    FP64 oracle (BF16 activations and weights: 3e-2 norm-wise, as the layered kernels) and
+    against the layered backward pass; the loss is added and the row offset advanced."""
+    from pyftle_amd.surrogate import FusedTrainStep
+
+    m = _train_model(width, depth, d_in, d_out)
+    p = m.to_numpy()
+    rng = np.random.default_rng(11)
+    M, extra = 1024, 128
+    lo, hi = [0, 0, 0, 0][:d_in], [2, 1, 10, 3][:d_in]
+    x = rng.uniform(lo, hi, (M + extra, d_in))
+    t = rng.standard_normal((M + extra, d_out)) * 0.3
+    X = torch.as_tensor(x, dtype=torch.float32, device="cuda")
+    Tn = torch.as_tensor(t / m.spec.out_scale, dtype=torch.float32, device="cuda")
+    base = torch.tensor([extra], dtype=torch.int64, device="cuda")  # rows extra .. extra + M - 1
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    step = FusedTrainStep(m, X, Tn, M, base, loss, splits=4)
+    step.grads()
+    torch.cuda.synchronize()
+    g_fused = m.grads.clone()
+    assert int(base.item()) == extra + M
+    xs, ts = x[extra:], t[extra:]
+    # the oracle's output includes out_scale; the kernel fits T / out_scale with the unscaled output
+    p_unscaled = dict(p, Wout=p["Wout"] / m.spec.out_scale, bout=p["bout"] / m.spec.out_scale)
+    l_ref, g_ref = MO.loss_and_grads(p_unscaled, xs, ts / m.spec.out_scale)
+    assert abs(loss.item() - l_ref) / l_ref < 3e-2
+    for l in range(depth):
+        o, shp, ob = m.offsets[l]
+        assert _rel(g_fused[o:o + shp[0] * shp[1]].view(shp).cpu().numpy(), g_ref["W"][l]) < 3e-2, l
+        assert _rel(g_fused[ob:ob + shp[0]].cpu().numpy(), g_ref["b"][l]) < 3e-2, l
+    o, shp, ob = m.out_off
+    assert _rel(g_fused[o:o + shp[0] * shp[1]].view(shp).cpu().numpy(), g_ref["Wout"]) < 3e-2
+    assert _rel(g_fused[ob:ob + d_out].cpu().numpy(), g_ref["bout"]) < 3e-2
+    # the layered backward pass on the same rows
+    m.loss_and_backward(X[extra:], Tn[extra:] * m.spec.out_scale)
+    torch.cuda.synchronize()
+    rel = ((m.grads - g_fused).norm() / m.grads.norm()).item()
+    print(f"[fused train {width}x{depth}] |g_fused - g_layered| / |g| = {rel:.2e}", flush=True)
+    assert rel < 2e-2
+
+
+def test_fused_training_reduces_loss_like_layered():
+    from pyftle_amd.surrogate import train
+
+    g = torch.Generator(device="cuda").manual_seed(0)
+    X = torch.rand(32768, 3, device="cuda", generator=g) * torch.tensor([2.0, 1.0, 10.0], device="cuda")
+    T = torch.stack([torch.sin(np.pi * X[:, 0]) * 0.3, torch.cos(np.pi * X[:, 1]) * 0.2 * torch.sin(X[:, 2])], 1)
+    out = {}
+    for fused in (True, False):
+        m = _train_model(256, 3, 3, 2, seed=3)
+        out[fused] = train(m, X, T, epochs=12, batch=2048, lr=3e-3, fused=fused)
+    print(f"[gpu] fused loss {out[True][0]:.3e} -> {out[True][-1]:.3e}; layered {out[False][-1]:.3e}", flush=True)
+    assert out[True][-1] < 0.1 * out[True][0]
+    assert out[True][-1] < 2.0 * out[False][-1]
