@@ -454,11 +454,19 @@ typedef struct pyftle_mlp_train_args {
   float *grads;
   float *loss;            /* device FP32 */
   void *work;
+  int32_t *step;          /* device step index, += 1 by the call (may be NULL) */
 } pyftle_mlp_train_args;
 int64_t pyftle_mlp_train_work_bytes(int32_t M, int32_t width, int32_t depth, int32_t d_out, int32_t splits);
 int pyftle_mlp_train_grads(const pyftle_mlp_train_args *args, void *stream);
 /* W_packed for pyftle_mlp_train_grads from the flat FP32 parameters. */
 int pyftle_mlp_pack_train(const float *params, int32_t width, int32_t depth, void *packed, void *stream);
+/* Adam as pyftle_mlp_adam_scheduled, with the learning rate of step *step (the step's
+ * index, which pyftle_mlp_train_grads advanced: start it at the first index minus one),
+ * writing W_packed for the next step instead of a flat bf16 copy. */
+int pyftle_mlp_train_adam(float *params, float *m, float *v, const float *grads, int32_t width, int32_t depth,
+                          int32_t d_out, float beta1, float beta2, float eps, float weight_decay, float lr_max,
+                          float lr_min, int32_t total_steps, int32_t adam_step0, const int32_t *step,
+                          void *packed, void *stream);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "mlp_gemm.hpp"
 
 namespace pyftle {
 
@@ -26,19 +27,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 enum { EPI_F32 = 0, EPI_BIAS_SILU = 1, EPI_DSILU = 2, EPI_BIAS = 3 };
 
-struct GemmArgs {
-  const __bf16 *A;
-  const __bf16 *B;
-  int64_t lda, ldb;   // leading dimension of the stored matrices
-  int M, N, K;
-  int k_per_split;    // K range of one split (grid z)
-  float *C32;         // EPI_F32: (split, M, N) partials / result
-  __bf16 *Cb;         // bf16 output
-  __bf16 *Z;          // EPI_BIAS_SILU: pre-activation out; EPI_DSILU: pre-activation in
-  int z_rows;         // EPI_DSILU: Z row = row % z_rows (tangent blocks stacked under primal rows)
-  const float *bias;  // (N,)
-  int64_t ldc;
-};
+// GemmArgs: mlp_gemm.hpp
 
 // Logistic sigmoid with the hardware exp2 and reciprocal (about 1 ulp each; results are
 // rounded to bf16 afterwards).  e^-z overflows to +inf for z < -88 and the reciprocal
@@ -91,7 +80,7 @@ struct TileLoader {
 };
 
 template <bool A_KC, bool B_KC, int EPI>
-__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
+__device__ __forceinline__ void gemm_tile(const GemmArgs &g, const int bx, const int by, const int bz) {
   // One LDS buffer: operand tiles in the main loop, the output tile after it.
   __shared__ __attribute__((aligned(16))) __bf16 smem[(BM + BN) * LDSK];
   __bf16(*As)[LDSK] = reinterpret_cast<__bf16(*)[LDSK]>(smem);
@@ -99,8 +88,8 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-  const int kb = blockIdx.z * g.k_per_split;
+  const int m0 = by * BM, n0 = bx * BN;
+  const int kb = bz * g.k_per_split;
   const int ke = min(g.K, kb + g.k_per_split);
   f32x16 acc[2][2];
 #pragma unroll
@@ -153,7 +142,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
         for (int e = 0; e < 16; ++e) {
           const int row = m0 + wm * 64 + i * 32 + (e >> 2) * 8 + (lane >> 5) * 4 + (e & 3);
           const int col = n0 + wn * 64 + j * 32 + (lane & 31);
-          g.C32[(int64_t)blockIdx.z * g.M * g.ldc + (int64_t)row * g.ldc + col] = acc[i][j][e];
+          g.C32[(int64_t)bz * g.M * g.ldc + (int64_t)row * g.ldc + col] = acc[i][j][e];
         }
   } else {
     // Stage the bf16 tile in LDS (the operand buffers are free after the
@@ -195,6 +184,42 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
       __syncthreads();
     }
   }
+}
+
+template <bool A_KC, bool B_KC, int EPI>
+__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
+  gemm_tile<A_KC, B_KC, EPI>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+struct GemmGroup {
+  GemmArgs g[kGemmGroupMax];
+  int start[kGemmGroupMax + 1];  // first work-group of each member
+  int n;
+};
+
+__global__ void __launch_bounds__(256) gemm_group_kernel(GemmGroup G) {
+  const int b = blockIdx.x;
+  int j = 0;
+  while (j + 1 < G.n && b >= G.start[j + 1]) ++j;
+  const GemmArgs &g = G.g[j];
+  const int nx = g.N / BN, ny = g.M / BM;
+  const int loc = b - G.start[j];
+  gemm_tile<true, true, EPI_F32>(g, loc % nx, (loc / nx) % ny, loc / (nx * ny));
+}
+
+int launch_gemm_group_f32(const GemmArgs *members, int n, hipStream_t st) {
+  if (n < 1 || n > kGemmGroupMax) return fail("gemm group: 1..8 members");
+  GemmGroup G{};
+  G.n = n;
+  int total = 0;
+  for (int j = 0; j < n; ++j) {
+    G.g[j] = members[j];
+    G.start[j] = total;
+    total += (members[j].N / BN) * (members[j].M / BM) * (members[j].K / members[j].k_per_split);
+  }
+  G.start[n] = total;
+  gemm_group_kernel<<<total, 256, 0, st>>>(G);
+  return check_launch("gemm group");
 }
 
 // ---------------------------------------------------------------------------

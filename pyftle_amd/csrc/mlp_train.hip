@@ -25,6 +25,7 @@
 // hit distinct banks, and rotates the words back in registers.
 
 #include "mlp_fused.hpp"
+#include "mlp_gemm.hpp"
 
 namespace pyftle {
 namespace {
@@ -388,54 +389,100 @@ struct ReduceArgs {
   int splits;
   const float *bias_part, *wout_part, *bout_part, *loss_part;
   int tiles, depth, W, O;
-  int64_t n_params;
+  int64_t n_weights;       // hidden-layer weights (the split-K part)
+  int n_tiled;             // biases, output weights and output biases (the per-tile part)
+  int weight_blocks;       // work-groups of the first part
   float *grads;
   float *loss;             // += the step's loss
   int64_t *base_row;       // += M
+  int *step;               // += 1 (may be NULL)
   int M;
 };
 
+// Gradients summed in a fixed order: the hidden weights over the GEMM splits (a thread per
+// weight), the rest over the row tiles (16 parameters per work-group, 16 interleaved partial
+// sums per parameter combined in order).
 __global__ void __launch_bounds__(256) train_reduce_kernel(ReduceArgs a) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) {
+  const int W = a.W;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     float l = 0.0f;
     for (int k = 0; k < a.tiles; ++k) l += a.loss_part[k];
     *a.loss += l;
     *a.base_row += a.M;
+    if (a.step) *a.step += 1;
   }
-  if (p >= a.n_params) return;
-  const int W = a.W;
-  int64_t off = 0, poff = 0;
-  for (int l = 0; l < a.depth; ++l) {
-    const int fan = l == 0 ? kFeat : W;
-    const int64_t nw = (int64_t)W * fan;
-    if (p < off + nw) {  // weight (o, i) of layer l
-      const int64_t loc = p - off;
-      const float *src = a.part + poff + loc;
-      float gsum = 0.0f;
-      for (int s = 0; s < a.splits; ++s) gsum += src[(int64_t)s * nw];
-      a.grads[p] = gsum;
-      return;
+  if ((int)blockIdx.x < a.weight_blocks) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.n_weights) return;
+    int64_t off = 0, poff = 0, woff = 0;  // flat offset, partial offset, weight index of layer l
+    for (int l = 0; l < a.depth; ++l) {
+      const int fan = l == 0 ? kFeat : W;
+      const int64_t nw = (int64_t)W * fan;
+      if (q < woff + nw) {
+        const int64_t loc = q - woff;
+        const float *src = a.part + poff + loc;
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+        int s = 0;
+        for (; s + 4 <= a.splits; s += 4) {
+          s0 += src[(int64_t)s * nw];
+          s1 += src[(int64_t)(s + 1) * nw];
+          s2 += src[(int64_t)(s + 2) * nw];
+          s3 += src[(int64_t)(s + 3) * nw];
+        }
+        for (; s < a.splits; ++s) s0 += src[(int64_t)s * nw];
+        a.grads[off + loc] = (s0 + s1) + (s2 + s3);
+        return;
+      }
+      off += nw + W;
+      poff += (int64_t)a.splits * nw;
+      woff += nw;
     }
-    if (p < off + nw + W) {  // bias o of layer l
-      const int o = (int)(p - off - nw);
-      float gsum = 0.0f;
-      for (int k = 0; k < a.tiles; ++k) gsum += a.bias_part[((int64_t)k * a.depth + l) * W + o];
-      a.grads[p] = gsum;
-      return;
+    return;
+  }
+  __shared__ float ps[16][16];
+  const int qi = threadIdx.x & 15, ph = threadIdx.x >> 4;
+  const int q = ((int)blockIdx.x - a.weight_blocks) * 16 + qi;
+  const float *src = nullptr;
+  int64_t stride = 0, dst = 0;
+  if (q < a.n_tiled) {
+    const int nb = a.depth * W;
+    int64_t out_off = 0;
+    for (int l = 0; l < a.depth; ++l) out_off += (int64_t)W * (l == 0 ? kFeat : W) + W;
+    if (q < nb) {
+      const int l = q / W, o = q % W;
+      src = a.bias_part + (int64_t)l * W + o;
+      stride = (int64_t)a.depth * W;
+      dst = (l == 0 ? (int64_t)W * kFeat : (int64_t)W * kFeat + W + (int64_t)(l - 1) * ((int64_t)W * W + W) +
+                                               (int64_t)W * W) + o;
+    } else if (q < nb + a.O * W) {
+      const int loc = q - nb;
+      src = a.wout_part + loc;
+      stride = (int64_t)a.O * W;
+      dst = out_off + loc;
+    } else {
+      const int o = q - nb - a.O * W;
+      src = a.bout_part + o;
+      stride = 4;
+      dst = out_off + (int64_t)a.O * W + o;
     }
-    off += nw + W;
-    poff += (int64_t)a.splits * nw;
   }
-  const int64_t loc = p - off;
-  float gsum = 0.0f;
-  if (loc < (int64_t)a.O * W) {  // output weight (o, f)
-    for (int k = 0; k < a.tiles; ++k) gsum += a.wout_part[(int64_t)k * a.O * W + loc];
-  } else {  // output bias
-    const int o = (int)(loc - (int64_t)a.O * W);
-    for (int k = 0; k < a.tiles; ++k) gsum += a.bout_part[k * 4 + o];
+  float s0 = 0.0f, s1 = 0.0f;
+  if (src) {
+    int k = ph;
+    for (; k + 16 < a.tiles; k += 32) {  // two independent chains keep two loads in flight
+      s0 += src[(int64_t)k * stride];
+      s1 += src[(int64_t)(k + 16) * stride];
+    }
+    if (k < a.tiles) s0 += src[(int64_t)k * stride];
   }
-  a.grads[p] = gsum;
+  ps[ph][qi] = s0 + s1;
+  __syncthreads();
+  if (ph == 0 && src) {
+    float t = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) t += ps[j][qi];
+    a.grads[dst] = t;
+  }
 }
 
 // index of element (m, k) of a rows x K matrix in the fragment-packed order of
@@ -462,6 +509,47 @@ __global__ void __launch_bounds__(256) pack_train_kernel(const float *params, in
       }
       return;
     }
+    off += nw + W;
+    pk += nw;
+  }
+}
+
+// Adam on the FP32 master weights with the cosine learning-rate schedule evaluated from
+// *step (the step's index, advanced by the reduction kernel; bias corrections for Adam step
+// t0 + *step + 1), writing the hidden weights' forward and transposed bf16 packings.
+__global__ void __launch_bounds__(256) train_adam_kernel(float *w, float *m, float *v, const float *g, int64_t n,
+                                                         int W, int depth, float b1, float b2, float eps, float wd,
+                                                         float lr_max, float lr_min, int total, int t0,
+                                                         const int *step, __bf16 *packed) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int k = *step;
+  const double c = cos(3.141592653589793 * (double)k / (double)(total > 0 ? total : 1));
+  const float lr = (float)((double)lr_min + 0.5 * ((double)lr_max - (double)lr_min) * (1.0 + c));
+  const float bc1 = 1.0f - powf(b1, (float)(t0 + k + 1));
+  const float bc2 = 1.0f - powf(b2, (float)(t0 + k + 1));
+  const float gi = g[p] + wd * w[p];
+  const float mi = b1 * m[p] + (1.0f - b1) * gi;
+  const float vi = b2 * v[p] + (1.0f - b2) * gi * gi;
+  m[p] = mi;
+  v[p] = vi;
+  const float wi = w[p] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  w[p] = wi;
+  int64_t off = 0, pk = 0;
+  for (int l = 0; l < depth; ++l) {
+    const int fan = l == 0 ? kFeat : W;
+    const int64_t nw = (int64_t)W * fan;
+    if (p < off + nw) {
+      const int64_t loc = p - off;
+      const int o = (int)(loc / fan), i = (int)(loc % fan);
+      packed[pk + packed_index(o, i, W)] = (__bf16)wi;
+      if (l > 0) {
+        const int64_t tb = (int64_t)W * kFeat + (int64_t)(depth - 1) * W * W + (int64_t)(l - 1) * W * W;
+        packed[tb + packed_index(i, o, fan)] = (__bf16)wi;
+      }
+      return;
+    }
+    if (p < off + nw + W) return;  // a bias
     off += nw + W;
     pk += nw;
   }
@@ -569,18 +657,31 @@ int pyftle_mlp_train_grads(const pyftle_mlp_train_args *t, void *stream) {
                                                             : launch_train<512, 4, 3>(a, st);
   }
   if (rc) return rc;
-  // weight gradients: dW_l (W x fan) = DZT_l (W x M) HT_{l-1}^T, both K-contiguous, split over K
+  // weight gradients: dW_l (W x fan) = DZT_l (W x M) HT_{l-1}^T, both K-contiguous, split over K,
+  // every layer in one launch
   float *part = reinterpret_cast<float *>(wk + wl.part);
-  int64_t poff = 0;
+  GemmArgs gm[kGemmGroupMax];
+  int64_t poff = 0, n_weights = 0, n_params = 0;
   for (int l = 0; l < depth; ++l) {
     const int fan = l == 0 ? mlpf::kFeat : W;
-    const __bf16 *A = a.DZT + (int64_t)l * W * t->M;
-    const __bf16 *B = l == 0 ? a.FT : a.HT + (int64_t)(l - 1) * W * t->M;
-    rc = pyftle_mlp_gemm(A, t->M, 1, B, t->M, 1, W, fan, t->M, S, 0, nullptr, nullptr, 0, part + poff, fan,
-                         stream);
-    if (rc) return rc;
+    GemmArgs &g = gm[l];
+    g = GemmArgs{};
+    g.A = a.DZT + (int64_t)l * W * t->M;
+    g.B = l == 0 ? a.FT : a.HT + (int64_t)(l - 1) * W * t->M;
+    g.lda = g.ldb = t->M;
+    g.M = W;
+    g.N = fan;
+    g.K = t->M;
+    g.k_per_split = t->M / S;
+    g.C32 = part + poff;
+    g.ldc = fan;
+    g.z_rows = W;
     poff += (int64_t)S * W * fan;
+    n_weights += (int64_t)W * fan;
+    n_params += (int64_t)W * fan + W;
   }
+  rc = launch_gemm_group_f32(gm, depth, st);
+  if (rc) return rc;
   ReduceArgs r{};
   r.part = part;
   r.splits = S;
@@ -592,15 +693,34 @@ int pyftle_mlp_train_grads(const pyftle_mlp_train_args *t, void *stream) {
   r.depth = depth;
   r.W = W;
   r.O = O;
-  int64_t n = 0;
-  for (int l = 0; l < depth; ++l) n += (int64_t)W * (l == 0 ? mlpf::kFeat : W) + W;
-  r.n_params = n + (int64_t)O * W + O;
+  r.n_weights = n_weights;
+  r.n_tiled = depth * W + O * W + O;
+  r.weight_blocks = (int)((n_weights + 255) / 256);
   r.grads = t->grads;
   r.loss = t->loss;
   r.base_row = t->base_row;
+  r.step = t->step;
   r.M = t->M;
-  train_reduce_kernel<<<(unsigned)((r.n_params + 255) / 256), 256, 0, st>>>(r);
+  const int blocks = r.weight_blocks + (r.n_tiled + 15) / 16;
+  train_reduce_kernel<<<blocks, 256, 0, st>>>(r);
+  (void)n_params;
   return check_launch("mlp_train_grads reduce");
+}
+
+int pyftle_mlp_train_adam(float *params, float *m, float *v, const float *grads, int32_t width, int32_t depth,
+                          int32_t d_out, float beta1, float beta2, float eps, float weight_decay, float lr_max,
+                          float lr_min, int32_t total_steps, int32_t adam_step0, const int32_t *step,
+                          void *packed, void *stream) {
+  if (!params || !m || !v || !grads || !step || !packed) return fail("mlp_train_adam: NULL pointer");
+  if ((width != 256 && width != 512) || depth < 2 || depth > mlpf::kMaxDepth || d_out < 1 || d_out > 3)
+    return fail("mlp_train_adam: width 256 or 512, depth in [2, 8], d_out in [1, 3]");
+  int64_t n = 0;
+  for (int l = 0; l < depth; ++l) n += (int64_t)width * (l == 0 ? mlpf::kFeat : width) + width;
+  n += (int64_t)d_out * width + d_out;
+  train_adam_kernel<<<(unsigned)((n + 255) / 256), 256, 0, as_stream(stream)>>>(
+      params, m, v, grads, n, width, depth, beta1, beta2, eps, weight_decay, lr_max, lr_min, total_steps,
+      adam_step0, step, static_cast<__bf16 *>(packed));
+  return check_launch("mlp_train_adam");
 }
 
 int pyftle_mlp_pack_train(const float *params, int32_t width, int32_t depth, void *packed, void *stream) {

@@ -48,6 +48,8 @@ _SIGS = {
     "pyftle_mlp_train_work_bytes": [_i32, _i32, _i32, _i32, _i32],
     "pyftle_mlp_train_grads": [_vp, _vp],
     "pyftle_mlp_pack_train": [_vp, _i32, _i32, _vp, _vp],
+    "pyftle_mlp_train_adam": [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _f32, _f32, _f32, _f32, _f32, _f32, _i32,
+                              _i32, _vp, _vp, _vp],
 }
 
 
@@ -56,7 +58,7 @@ class TrainArgs(ctypes.Structure):
 
     _fields_ = [("x", _vp), ("targets", _vp), ("base_row", _vp), ("M", _i32), ("d_in", _i32), ("d_out", _i32),
                 ("n_freq", _i32), ("width", _i32), ("depth", _i32), ("splits", _i32), ("scale", _vp), ("Bf", _vp),
-                ("W_packed", _vp), ("params", _vp), ("grads", _vp), ("loss", _vp), ("work", _vp)]
+                ("W_packed", _vp), ("params", _vp), ("grads", _vp), ("loss", _vp), ("work", _vp), ("step", _vp)]
 # (outputs, tangents) covered by the one-kernel network, per hidden width
 _FUSED_SHAPES = {256: ((2, 0), (2, 2), (3, 0)), 512: ((2, 0), (3, 0), (3, 3))}
 _registered = False
@@ -368,11 +370,12 @@ def fused_train_supported(spec: MLPSpec, batch: int) -> bool:
 class FusedTrainStep:
     """One minibatch step on the fused training kernels: gradients of rows
     ``*base .. *base + batch - 1`` of (X, T / out_scale) (``pyftle_mlp_train_grads``, which
-    also adds the loss to ``loss`` and advances ``base``), the scheduled Adam update and the
-    repacking of the bf16 weights the next step reads (``pyftle_mlp_pack_train``)."""
+    also adds the loss to ``loss``, advances ``base`` and, given, the step index ``step``),
+    then ``pyftle_mlp_train_adam`` (Adam at that step's learning rate, writing the packed
+    bf16 weights the next step reads)."""
 
     def __init__(self, model: FlowMapMLP, X: torch.Tensor, Tn: torch.Tensor, batch: int, base: torch.Tensor,
-                 loss: torch.Tensor, splits: int = 0):
+                 loss: torch.Tensor, splits: int = 0, step: torch.Tensor | None = None):
         _lib()
         s = model.spec
         if not fused_train_supported(s, batch):
@@ -397,8 +400,9 @@ class FusedTrainStep:
         a.scale, a.Bf = model.scale.data_ptr(), model.B.data_ptr()
         a.W_packed, a.params, a.grads = self.packed.data_ptr(), model.params.data_ptr(), model.grads.data_ptr()
         a.loss, a.work = loss.data_ptr(), self.work.data_ptr()
+        a.step = step.data_ptr() if step is not None else None
         self.args = a
-        self._keep = (X, Tn, base, loss)
+        self._keep = (X, Tn, base, loss, step)
 
     def repack(self):
         s = self.model.spec
@@ -406,6 +410,12 @@ class FusedTrainStep:
 
     def grads(self):
         N.call("pyftle_mlp_train_grads", ctypes.byref(self.args), stream_handle())
+
+    def adam(self, lr_max: float, lr_min: float, total: int, step0: int, step: torch.Tensor):
+        m, s = self.model, self.model.spec
+        N.call("pyftle_mlp_train_adam", _p(m.params), _p(m.m), _p(m.v), _p(m.grads), s.width, s.depth, s.d_out,
+               0.9, 0.999, 1e-8, 0.0, float(lr_max), float(lr_min), int(total), int(step0), _p(step),
+               _p(self.packed), stream_handle())
 
 
 def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20, batch: int = 32768,
@@ -442,9 +452,11 @@ def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20,
 
     use_fused = fused and fused_train_supported(model.spec, batch)
     if use_fused:
-        # targets of the network output, shuffled with the inputs each epoch
+        # targets of the network output, shuffled with the inputs each epoch; the step index
+        # is advanced by the gradient call before Adam reads it
         Tn = (T / model.spec.out_scale).contiguous()
-        fstep = FusedTrainStep(model, Xs, Ts, batch, base, acc)
+        k_dev.fill_(-1)
+        fstep = FusedTrainStep(model, Xs, Ts, batch, base, acc, step=k_dev)
     else:
         Tn = T
 
@@ -455,9 +467,8 @@ def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20,
 
     def body():
         if use_fused:
-            fstep.grads()  # also advances base
-            adam()
-            fstep.repack()
+            fstep.grads()  # also advances base and the step index
+            fstep.adam(lr, lr_min, total, step0, k_dev)
             return
         idx = base + ar
         torch.index_select(Xs, 0, idx, out=xb)
@@ -489,6 +500,8 @@ def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20,
         if log:
             log(f"epoch {ep} loss {losses[-1]:.3e}")
     model.step += total
+    if use_fused:
+        model.sync_bf16()  # the fused steps wrote the packed weights only
     return losses
 
 
