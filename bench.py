@@ -692,19 +692,30 @@ def run_surrogate3d_infer(torch, wl, args):
     print(json.dumps(result), flush=True)
 
 
-def build_mlp_train(torch, batch=16384, depth=5, width=256, n_rows=1 << 21):
-    """One surrogate member (the 2D network: 64 Fourier frequencies on (x, y, t0), depth x
-    width SiLU, 2 outputs) trained on synthetic smooth targets, to time a training step."""
+def build_mlp_train(torch, batch=16384, shape="2d", n_rows=1 << 21):
+    """One surrogate member trained on synthetic smooth targets, to time a training step:
+    the 2D network (64 Fourier frequencies on (x, y, t0), 5 x 256 SiLU, 2 outputs) or the 3D
+    one (on (x, y, z, t0), 6 x 512 SiLU, 3 outputs)."""
     from pyftle_amd import surrogate as S
 
     g = torch.Generator(device="cuda").manual_seed(0)
-    X = torch.rand(n_rows, 3, generator=g, device="cuda") * torch.tensor([2.0, 1.0, 10.0], device="cuda")
-    D = torch.stack([0.1 * torch.sin(3.0 * X[:, 0] + X[:, 2]) * torch.cos(2.0 * X[:, 1]),
-                     0.1 * torch.cos(2.0 * X[:, 0]) * torch.sin(3.0 * X[:, 1] - 0.5 * X[:, 2])], dim=1)
-    spec = S.MLPSpec(sigma=1.0, depth=depth, width=width)
+    if shape == "3d":
+        X = torch.rand(n_rows, 4, generator=g, device="cuda") * torch.tensor([6.28, 6.28, 6.28, 10.0], device="cuda")
+        D = torch.stack([0.5 * torch.sin(X[:, 2] + 0.2 * X[:, 3]) + 0.3 * torch.cos(X[:, 1]),
+                         0.5 * torch.cos(X[:, 2]) + 0.4 * torch.sin(X[:, 0]),
+                         0.3 * torch.sin(X[:, 1]) + 0.4 * torch.cos(X[:, 0] - 0.1 * X[:, 3])], dim=1)
+        spec = S.MLPSpec(d_in=4, d_out=3, n_freq=64, width=512, depth=6, sigma=0.5, scale=(0.5, 0.5, 0.5, 0.1),
+                         out_scale=2.0)
+        mlp = "64 Fourier freqs on (x,y,z,t0), 6x512 SiLU -> 3"
+    else:
+        X = torch.rand(n_rows, 3, generator=g, device="cuda") * torch.tensor([2.0, 1.0, 10.0], device="cuda")
+        D = torch.stack([0.1 * torch.sin(3.0 * X[:, 0] + X[:, 2]) * torch.cos(2.0 * X[:, 1]),
+                         0.1 * torch.cos(2.0 * X[:, 0]) * torch.sin(3.0 * X[:, 1] - 0.5 * X[:, 2])], dim=1)
+        spec = S.MLPSpec(sigma=1.0, depth=5, width=256)
+        mlp = "64 Fourier freqs on (x,y,t0), 5x256 SiLU -> 2"
     return dict(name="mlp_train", X=X, D=D.contiguous(), spec=spec, batch=batch, S=S,
-                config={"workload": "mlp_train", "mlp": f"64 Fourier freqs on (x,y,t0), {depth}x{width} SiLU -> 2",
-                        "batch": batch, "rows": n_rows, "data": "synthetic smooth targets"})
+                config={"workload": "mlp_train", "mlp": mlp, "batch": batch, "rows": n_rows,
+                        "data": "synthetic smooth targets"})
 
 
 def run_mlp_train(torch, wl, args):
@@ -714,12 +725,13 @@ def run_mlp_train(torch, wl, args):
     spec, batch = wl["spec"], wl["batch"]
     m = S.FlowMapMLP(spec, seed=1)
     epochs = max(2, args.steps)
-    S.train(m, wl["X"], wl["D"], epochs=1, batch=batch, lr=3e-3, graph=args.train_graph)  # warm-up epoch
+    fused = not args.train_layered
+    S.train(m, wl["X"], wl["D"], epochs=1, batch=batch, lr=3e-3, graph=args.train_graph, fused=fused)  # warm-up
     torch.cuda.synchronize()
     a = torch.cuda.Event(enable_timing=True)
     b = torch.cuda.Event(enable_timing=True)
     a.record()
-    losses = S.train(m, wl["X"], wl["D"], epochs=epochs, batch=batch, lr=3e-3, graph=args.train_graph)
+    losses = S.train(m, wl["X"], wl["D"], epochs=epochs, batch=batch, lr=3e-3, graph=args.train_graph, fused=fused)
     b.record()
     torch.cuda.synchronize()
     ms = a.elapsed_time(b)
@@ -729,7 +741,8 @@ def run_mlp_train(torch, wl, args):
     log(f"{steps} steps of {batch} rows: {ms / steps:.3f} ms/step, {tf:.1f} TFLOP/s, loss {losses[-1]:.3e}")
     print(json.dumps({"metric": "surrogate training TFLOP/s", "value": tf, "unit": "TFLOP/s", "n_gpus": 1,
                       "steps": steps, "ms_per_step": ms / steps, "higher_is_better": True, "dtype": "bf16",
-                      "data": "synthetic", "config": dict(wl["config"], graph=bool(args.train_graph)),
+                      "data": "synthetic", "config": dict(wl["config"], graph=bool(args.train_graph),
+                                                          kernels="layered" if args.train_layered else "fused"),
                       "mfma": {"achieved": tf, "peak": 2500.0, "unit": "TFLOP/s", "frac": tf / 2500.0,
                                "flops_per_row": S.mlp_train_flops_per_row(spec)},
                       "final_loss": losses[-1]}), flush=True)
@@ -1043,6 +1056,8 @@ def main():
                          "other depths are an accuracy/speed exploration)")
     ap.add_argument("--train-batch", type=int, default=0, help="mlp_train: rows per step (0 = 16384)")
     ap.add_argument("--train-graph", type=int, default=0, help="mlp_train: replay each step as a HIP graph")
+    ap.add_argument("--train-shape", default="2d", choices=["2d", "3d"], help="mlp_train: the 2D or the 3D network")
+    ap.add_argument("--train-layered", action="store_true", help="mlp_train: the layer-by-layer kernels")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy check")
     ap.add_argument("--seed-order", default="auto", choices=["auto", "morton", "none"],
@@ -1088,8 +1103,10 @@ def main():
         kw["epochs"] = args.surrogate_epochs
     if args.workload == "surrogate2d" and args.surrogate_depth > 0:
         kw["depth"] = args.surrogate_depth
-    if args.workload == "mlp_train" and args.train_batch > 0:
-        kw["batch"] = args.train_batch
+    if args.workload == "mlp_train":
+        kw["shape"] = args.train_shape
+        if args.train_batch > 0:
+            kw["batch"] = args.train_batch
     global RECORDS
     RECORDS = args.records
     wl = WORKLOADS[args.workload](torch, **kw)

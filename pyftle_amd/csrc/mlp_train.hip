@@ -232,6 +232,7 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
           *reinterpret_cast<bf16x4_t *>(act + off + t * 16 * W) = h;
           *reinterpret_cast<bf16x4_t *>(dact + ((i * RT + t) * 64 + ln) * 4) = d;
         }
+        __builtin_amdgcn_sched_barrier(0);  // one feature tile at a time: no loads hoisted
       }
     }
     __syncthreads();
@@ -270,6 +271,7 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
 #pragma unroll
             for (int o = 0; o < O; ++o) yp[t][o] = fmaf(hz, wv[o][e], yp[t][o]);
           }
+        __builtin_amdgcn_sched_barrier(0);
       }
 #pragma unroll
       for (int t = 0; t < RT; ++t)
@@ -358,6 +360,7 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
         for (int o = 0; o < O; ++o)
           *reinterpret_cast<f32x4_t *>(wpart + o * W + f0) = f32x4_t{ws[o][0], ws[o][1], ws[o][2], ws[o][3]};
       }
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
   // backward through the hidden layers: acc holds dZ of layer l
