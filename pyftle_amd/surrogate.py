@@ -511,7 +511,8 @@ class FlowMapEnsemble:
     def __init__(self, members: list[FlowMapMLP]):
         self.members = members
 
-    def predict(self, x: torch.Tensor, dim: int, fused: bool = True, jacobian_members: int | None = None):
+    def predict(self, x: torch.Tensor, dim: int, fused: bool = True, jacobian_members: int | None = None,
+                per_member: bool = False):
         """Mean displacement, mean Jacobian of the displacement and member spread.
 
         The spread is the largest deviation of a member's displacement from the
@@ -540,6 +541,8 @@ class FlowMapEnsemble:
         Jd = torch.stack(jacs)
         mean = Y.mean(0)
         spread = (Y - mean).abs().amax(dim=(0, 2))
+        if per_member:  # the members' displacement Jacobians, (members, M, d_out, dim)
+            return mean, Jd, spread
         return mean, Jd.mean(0), spread
 
 

@@ -9,9 +9,17 @@ Pipeline:
    forward/backward/Adam kernels.
 3. ``SurrogateFTLE.ftle``: per seed the members' displacement and its
    Jacobian (JVP tangents) -> FTLE of ``J = I + mean dD/dx`` with the FP64
-   closed-form kernels; seeds whose member spread exceeds ``tol`` are redone by
-   the fused RK4 window kernel (the fallback), so the returned field is RK4
-   wherever the ensemble is unsure.
+   closed-form kernels; seeds where the members disagree by more than ``tol`` are
+   redone by the fused RK4 window kernel (the fallback), so the returned field is
+   RK4 wherever the ensemble is unsure.
+
+The disagreement (``gate``):
+* ``"ftle"`` (default): every member's own FTLE (from its own JVP Jacobian) against the
+  FTLE of the ensemble's mean Jacobian, ``max_m |ftle_m - ftle|``, in units of the
+  window's FTLE range (``max - min`` of the ensemble field over the queried seeds) --
+  the quantity the accuracy bar is stated in;
+* ``"displacement"``: the largest deviation of a member's displacement from the mean,
+  in domain units.
 """
 
 from __future__ import annotations
@@ -85,8 +93,11 @@ class SurrogateFTLE:
 
     def __init__(self, ensemble: FlowMapEnsemble, field: DeviceField, T: float, h: float,
                  delta: float = 1e-3, tol: float = 1e-3, time_mode: str = "linear", chunk: int = 1 << 21,
-                 jacobian_members: int | None = None):
+                 jacobian_members: int | None = None, gate: str = "ftle"):
+        if gate not in ("ftle", "displacement"):
+            raise ValueError(f"unknown gate {gate!r}: 'ftle' or 'displacement'")
         self.ens = ensemble
+        self.gate = gate
         self.jacobian_members = jacobian_members
         self.field = field
         self.T = T
@@ -97,7 +108,7 @@ class SurrogateFTLE:
         self.chunk = chunk
 
     def predict(self, seeds: torch.Tensor, t0: float):
-        """Surrogate-only FTLE, mean displacement Jacobian and spread for ``seeds`` (N, dim)."""
+        """Surrogate-only FTLE and the members' disagreement (``gate``) for ``seeds`` (N, dim)."""
         dim = seeds.shape[1]
         out_f, out_s = [], []
         for s0 in range(0, seeds.shape[0], self.chunk):
@@ -105,10 +116,26 @@ class SurrogateFTLE:
             x = torch.empty(sd.shape[0], dim + 1, dtype=torch.float32, device=sd.device)
             x[:, :dim] = sd.float()
             x[:, dim] = t0
-            _, Jd, spread = self.ens.predict(x, dim, jacobian_members=self.jacobian_members)
-            out_f.append(ftle_from_displacement_jacobian(Jd, self.T))
-            out_s.append(spread)
-        return torch.cat(out_f), torch.cat(out_s)
+            if self.gate == "ftle":
+                _, Jm, _ = self.ens.predict(x, dim, per_member=True)
+                f = ftle_from_displacement_jacobian(Jm.mean(0), self.T)
+                dev = torch.zeros_like(f)
+                for k in range(Jm.shape[0]):
+                    torch.maximum(dev, (ftle_from_displacement_jacobian(Jm[k], self.T) - f).abs(), out=dev)
+                out_f.append(f)
+                out_s.append(dev)
+            else:
+                _, Jd, spread = self.ens.predict(x, dim, jacobian_members=self.jacobian_members)
+                out_f.append(ftle_from_displacement_jacobian(Jd, self.T))
+                out_s.append(spread)
+        f = torch.cat(out_f)
+        spread = torch.cat(out_s)
+        if self.gate == "ftle":  # in units of the window's FTLE range
+            fin = f[torch.isfinite(f)]
+            rng = (fin.max() - fin.min()).clamp_min(1e-300) if fin.numel() else torch.ones((), device=f.device)
+            spread = spread / rng
+            spread[~torch.isfinite(spread)] = float("inf")
+        return f, spread
 
     def rk4_ftle(self, seeds: torch.Tensor, t0: float) -> torch.Tensor:
         """Plain RK4 FTLE of the same seeds (neighbour groups at +-delta), with the launch

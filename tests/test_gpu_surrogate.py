@@ -6,7 +6,8 @@
   (plain RK4 on the same seeds) in time per field at 5% RMSE (measured 2.8x at
   1.4-1.7%); the full-size measurement is ``bench.py --workload surrogate2d``;
 * a member perturbed on purpose disagrees everywhere, so every seed falls back
-  and the field equals the RK4 field exactly;
+  and the field equals the RK4 field exactly; the FTLE gate's disagreement is zero
+  for identical members and flags a member whose Jacobian differs;
 * on a linear flow the JVP Jacobian of a network fitted to the exact
   displacement gives the closed-form FTLE.
 """
@@ -84,7 +85,8 @@ def test_fallback_recomputes_disagreeing_seeds_with_rk4(gyre_setup):
     bad.params[ob:ob + shp[0]] += 0.5  # shift one member's output: it disagrees everywhere
     bad.sync_bf16()
     try:
-        sf = SF.SurrogateFTLE(FlowMapEnsemble(ens.members), f, T, h, tol=1e-3)
+        # a constant output shift leaves the Jacobians alone: the displacement gate sees it
+        sf = SF.SurrogateFTLE(FlowMapEnsemble(ens.members), f, T, h, tol=1e-3, gate="displacement")
         seeds = _seeds(64, 32)
         res = sf.ftle(seeds, 0.7)
         assert bool(res.fallback.all())
@@ -207,3 +209,32 @@ def test_trained_network_gains_over_rk4_at_fixed_accuracy():
               f"FTLE rmse/range {errs[-1]:.4f}", flush=True)
     assert max(errs) < 0.05
     assert min(gains) > 1.5
+
+
+def test_ftle_gate_units_and_forced_fallback(gyre_setup):
+    """The FTLE gate: identical members agree exactly (disagreement 0); a member whose
+    first hidden layer is scaled disagrees, in units of the window's FTLE range; a negative
+    tolerance forces every seed to RK4, bit for bit."""
+    from pyftle_amd import surrogate_ftle as SF
+    from pyftle_amd.surrogate import FlowMapEnsemble
+
+    f, ens, T, h = gyre_setup
+    seeds = _seeds(64, 32)
+    same = SF.SurrogateFTLE(FlowMapEnsemble([ens.members[0], ens.members[0]]), f, T, h, gate="ftle")
+    fs, dis = same.predict(seeds, 0.5)
+    assert float(dis.max()) == 0.0
+    sf = SF.SurrogateFTLE(ens, f, T, h, gate="ftle", tol=-1.0)
+    res = sf.ftle(seeds, 0.5)
+    assert bool(res.fallback.all()) and torch.equal(res.ftle, sf.rk4_ftle(seeds, 0.5))
+    bad = ens.members[1]
+    saved = bad.params.clone()
+    o, shp, ob = bad.offsets[0]
+    bad.params[o:o + shp[0] * shp[1]] *= 1.5
+    bad.sync_bf16()
+    try:
+        _, dis2 = SF.SurrogateFTLE(FlowMapEnsemble(ens.members), f, T, h, gate="ftle").predict(seeds, 0.5)
+        print(f"[gpu] perturbed member: FTLE disagreement / range median {dis2.median().item():.3e}", flush=True)
+        assert float(dis2.median()) > 1e-2
+    finally:
+        bad.params.copy_(saved)
+        bad.sync_bf16()
