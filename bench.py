@@ -275,8 +275,9 @@ def surrogate2d_measure(torch, wl, args):
         torch.cuda.synchronize()
         return out, a.elapsed_time(b) / reps
 
+    reps = max(1, min(3, args.steps))  # timed repetitions (the work is ~1 s per pass)
     plan, wins = SF.rk4_ftle_windows(field, seeds, t0s, wl["T"], wl["h"])
-    _, rk4_ms = timed(lambda: plan.run(), max(1, args.steps))
+    _, rk4_ms = timed(lambda: plan.run(), reps)
     log(f"plain RK4: {rk4_ms:.1f} ms for {n_sw} seed-windows")
     fpq = S.mlp_flops_per_query(spec)
     M = wl["members"]
@@ -290,7 +291,7 @@ def surrogate2d_measure(torch, wl, args):
     alone = {}
     for gate, jm in (("ftle", M), ("displacement", 1)):
         sf = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], gate=gate, jacobian_members=jm)
-        preds, ms = timed(lambda: [sf.predict(seeds, t0) for t0 in t0s], max(1, args.steps))
+        preds, ms = timed(lambda: [sf.predict(seeds, t0) for t0 in t0s], reps)
         rows = n_sw * (jm * 3 + (M - jm))
         mean, worst = rmse_of([p[0] for p in preds])
         sp = torch.cat([p[1] for p in preds])
@@ -1099,6 +1100,8 @@ def main():
     ap.add_argument("--train-shape", default="2d", choices=["2d", "3d"], help="mlp_train: the 2D or the 3D network")
     ap.add_argument("--train-layered", action="store_true", help="mlp_train: the layer-by-layer kernels")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="abc128: skip the surrogate leg (training and timing the 2D surrogate, ~1 min)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy check")
     ap.add_argument("--seed-order", default="auto", choices=["auto", "morton", "none"],
                     help="abc128 / gyre512: process seed groups in Morton order of their cells (results are "
@@ -1276,6 +1279,14 @@ def main():
     if not args.no_cpu:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(wl)
+    if wl["name"] == "abc128" and not args.no_surrogate:
+        # the metric's surrogate half: the 2D flow-map surrogate against plain GPU RK4
+        log("surrogate leg (2D double gyre, 4 members of 5x256, 67.1M seed-windows)")
+        del wl
+        torch.cuda.empty_cache()
+        summ, plan = surrogate2d_measure(torch, build_surrogate2d(torch), args)
+        plan.close()
+        result["surrogate"] = summ
     print(json.dumps(result), flush=True)
 
 
