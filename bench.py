@@ -176,13 +176,20 @@ def _gyre_field_on_device(torch, nx, ny, n_snap, eps, omega, dt_snap=0.1, A=0.1,
     x = torch.linspace(0.0, 2.0, nx, dtype=torch.float64, device=dev)
     y = torch.linspace(0.0, 1.0, ny, dtype=torch.float64, device=dev)
     Y, X = torch.meshgrid(y, x, indexing="ij")  # flat ix + nx*iy
-    for k in range(n_snap):
-        sn = math.sin(omega * dt_snap * k)
-        f = eps * sn * X * X + (1.0 - 2.0 * eps * sn) * X
-        dfdx = 2.0 * eps * sn * X + (1.0 - 2.0 * eps * sn)
+    # every snapshot in one batch of element-wise kernels (a few dozen dispatches per field,
+    # not a few dozen per snapshot), then one pack per snapshot; chunks bound the memory
+    sn_all = torch.tensor([math.sin(omega * dt_snap * k) for k in range(n_snap)], dtype=torch.float64, device=dev)
+    chunk = max(1, (1 << 26) // (nx * ny))
+    for k0 in range(0, n_snap, chunk):
+        sn = sn_all[k0:k0 + chunk][:, None, None]
+        f = (eps * sn) * X * X + (1.0 - 2.0 * eps * sn) * X
+        dfdx = (2.0 * eps * sn) * X + (1.0 - 2.0 * eps * sn)
         u = -math.pi * A * torch.sin(math.pi * f) * torch.cos(math.pi * Y)
         v = math.pi * A * torch.cos(math.pi * f) * torch.sin(math.pi * Y) * dfdx
-        field.pack(k, torch.stack([u.reshape(-1), v.reshape(-1)]))
+        uv = torch.stack([u.reshape(u.shape[0], -1), v.reshape(v.shape[0], -1)], dim=1)  # (chunk, 2, n)
+        for j in range(uv.shape[0]):
+            field.pack(k0 + j, uv[j])
+        del f, dfdx, u, v, uv
     return field
 
 
