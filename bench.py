@@ -768,6 +768,71 @@ def build_mlp_train(torch, batch=16384, shape="2d", n_rows=1 << 21):
                         "data": "synthetic smooth targets"})
 
 
+def run_tile_occupancy(torch, wl, args):
+    """Ceiling of an LDS tile path for the advection kernel: the share of particles that are
+    still inside their work-group's tile -- the box of cells (as many as one work-group's
+    share of LDS holds: 80 KB at two work-groups per CU) centred on the work-group's initial
+    cell -- as the window advances.  Only samples in the tile could be served from LDS
+    instead of the texture path, so this fraction bounds the TD time such a path could save.
+    Work-groups are the bench's launch: the tuned number of seed groups per work-group in
+    Morton order.  The window runs in chunks of steps through pyftle_advect; positions are
+    compared with the tiles between chunks."""
+    from pyftle_amd import window as W
+    from pyftle_amd.tuning import best_config, shape_class
+
+    field, dim = wl["field"], wl["dim"]
+    nb = 2 * dim
+    n = wl["n_seeds"]
+    cfg = best_config(shape_class(dim, n, wl["time_mode"], records=_records_name(dim)))
+    G = cfg.groups_per_block
+    rec = 96 if dim == 3 else 64  # bytes per cell record (3D single level, 2D time pair)
+    side = int((80 * 1024 // rec) ** (1.0 / dim))
+    order = W.seed_order(field, wl["pos"], nb).long()
+    geo = field.geom
+    org = torch.tensor(geo.origin[:dim], dtype=torch.float64, device="cuda")
+    spc = torch.tensor(geo.spacing[:dim], dtype=torch.float64, device="cuda")
+    per = torch.tensor([bool(p) for p in (geo.periodic or (False,) * dim)][:dim], device="cuda")
+    ncell = torch.tensor(geo.n[:dim], dtype=torch.float64, device="cuda")
+
+    def cells(p):
+        return torch.floor((p - org) / spc)
+
+    pos = wl["pos"].clone()
+    # particle j of seed group s is row j * n + s; the work-group of a group is its rank // G
+    rank = torch.empty(n, dtype=torch.int64, device="cuda")
+    rank[order] = torch.arange(n, device="cuda")
+    wg = rank // G
+    c0 = cells(pos.view(nb, n, dim).mean(0))
+    n_wg = int(wg.max().item()) + 1
+    centre = torch.zeros(n_wg, dim, dtype=torch.float64, device="cuda")
+    centre.index_add_(0, wg, c0)
+    centre = torch.floor(centre / torch.bincount(wg, minlength=n_wg)[:, None].double())
+    ctr = centre[wg]  # (n, dim) per seed group
+    half = side // 2
+    chunk = max(1, wl["steps"] // 20)
+    rows = []
+    done = 0
+    while done < wl["steps"]:
+        k = min(chunk, wl["steps"] - done)
+        if wl["time_mode"] == "steady":
+            W.advect(field, pos, method="rk4", time_mode="steady", n_steps=k, h=wl["h"])
+        else:
+            W.advect(field, pos, method="rk4", time_mode="linear", n_steps=k, h=wl["h"],
+                     t_start=wl["t_start"] + done * wl["h"])
+        done += k
+        d = cells(pos.view(nb, n, dim)) - ctr[None]
+        d = torch.where(per, torch.remainder(d + ncell / 2, ncell) - ncell / 2, d)
+        inside = (d.abs() <= half).all(dim=2).float().mean().item()
+        rows.append({"step": done, "inside": inside})
+    torch.cuda.synchronize()
+    mean_inside = sum(r["inside"] for r in rows) / len(rows)
+    log(f"tile {side}^{dim} cells: inside fraction over the window " +
+        ", ".join(f"{r['step']}:{r['inside']:.3f}" for r in rows))
+    print(json.dumps({"workload": wl["name"], "tile_cells_per_axis": side, "record_bytes": rec,
+                      "groups_per_block": G, "seed_order": "morton", "chunk_steps": chunk,
+                      "inside_fraction_by_step": rows, "mean_inside_fraction": mean_inside}), flush=True)
+
+
 def run_mlp_train(torch, wl, args):
     """Training steps of one member: forward, backward and Adam per minibatch, timed with
     HIP events over whole epochs; flops = forward + input gradients + weight gradients."""
@@ -1106,6 +1171,9 @@ def main():
     ap.add_argument("--train-graph", type=int, default=0, help="mlp_train: replay each step as a HIP graph")
     ap.add_argument("--train-shape", default="2d", choices=["2d", "3d"], help="mlp_train: the 2D or the 3D network")
     ap.add_argument("--train-layered", action="store_true", help="mlp_train: the layer-by-layer kernels")
+    ap.add_argument("--tile-occupancy", action="store_true",
+                    help="abc128 / gyre512: only measure how much of the window an LDS tile per work-group "
+                         "could serve (share of particles inside their work-group's tile, by step)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="abc128: skip the surrogate leg (training and timing the 2D surrogate, ~1 min)")
@@ -1166,6 +1234,8 @@ def main():
         return run_batch(torch, wl, args)
     if wl["name"] == "mlp_train":
         return run_mlp_train(torch, wl, args)
+    if args.tile_occupancy:
+        return run_tile_occupancy(torch, wl, args)
     if wl["name"] == "surrogate2d":
         return run_surrogate(torch, wl, args)
     if wl["name"] == "surrogate_infer":
