@@ -24,7 +24,8 @@
 //
 // Per layer: the MFMAs over K (the weight fragments of the next k-step in flight; the
 // next layer's first ones are issued during the last k-step), a barrier, the epilogue
-// SiLU(z + b) written back to LDS in place as bf16 at raised wave priority, a barrier.
+// SiLU(z + b) written back to LDS in place as bf16 (scaled by -log2(e); mlp_fused.hpp
+// gives the bookkeeping) at raised wave priority, a barrier.
 // Two work-groups share a CU (57-75 KB of LDS each), so one's epilogue runs under the
 // other's MFMAs.  The output layer (O <= 3) is FP32 on the last hidden layer's registers,
 // reduced over lanes and waves.
@@ -36,7 +37,9 @@ namespace {
 
 using namespace mlpf;
 
-// Last hidden layer: this lane's partial sums of the output layer, in FP32.
+// Last hidden layer: this lane's partial sums of the output layer, in FP32, with c Wout on
+// the scaled activations (the primal rows come out exact; the tangent rows still carry
+// the factor c^-depth).
 template <int W, int RT, int NT, int O>
 __device__ __forceinline__ void output_partials(const f32x4_t (&acc)[Shape<W, RT>::kTilesPerWave][RT],
                                                 const float *bias_l, const float *wout, int ft0, int lane_in,
@@ -54,29 +57,34 @@ __device__ __forceinline__ void output_partials(const f32x4_t (&acc)[Shape<W, RT
     const f32x4_t bv = *reinterpret_cast<const f32x4_t *>(bias_l + f0);
     f32x4_t wv[O];
 #pragma unroll
-    for (int o = 0; o < O; ++o) wv[o] = *reinterpret_cast<const f32x4_t *>(wout + o * W + f0);
+    for (int o = 0; o < O; ++o) wv[o] = *reinterpret_cast<const f32x4_t *>(wout + o * W + f0) * kNegLn2;
 #pragma unroll
     for (int q = 0; q < G; ++q) {
       const int b0 = q * (1 + NT);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const float z = acc[i][b0][e] + bv[e];
-        const float s = sigmoid_f(z);
-        const float hz = z * s;
-        const float d = s * (1.0f + z * (1.0f - s));
+        const float z = fmaf(bv[e], kNegLog2e, acc[i][b0][e]);
+        const float s = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z));
+        const float hp = z * s;
 #pragma unroll
-        for (int o = 0; o < O; ++o) {
-          yp[b0][o] = fmaf(hz, wv[o][e], yp[b0][o]);
+        for (int o = 0; o < O; ++o) yp[b0][o] = fmaf(hp, wv[o][e], yp[b0][o]);
+        if constexpr (NT > 0) {
+          const float dd = fmaf(s, kNegLog2e - hp, hp);
 #pragma unroll
-          for (int a = 1; a <= NT; ++a) yp[b0 + a][o] = fmaf(acc[i][b0 + a][e] * d, wv[o][e], yp[b0 + a][o]);
+          for (int a = 1; a <= NT; ++a) {
+            const float tv = acc[i][b0 + a][e] * dd;
+#pragma unroll
+            for (int o = 0; o < O; ++o) yp[b0 + a][o] = fmaf(tv, wv[o][e], yp[b0 + a][o]);
+          }
         }
       }
     }
   }
 }
 
-// Fourier features of the tile's seeds (and their input derivatives) into LDS rows:
-// features [sin | cos] of the 64 frequencies, one task per (seed, block of 8 frequencies).
+// Fourier features of the tile's seeds (and their input derivatives), scaled by -log2(e),
+// into LDS rows: features [sin | cos] of the 64 frequencies, one task per (seed, block of 8
+// frequencies); dfreq_s already holds 2 pi B scale (-log2(e)).
 template <int W, int RT, int NT>
 __device__ __forceinline__ void fourier_rows(const FusedArgs &a, int64_t tile, const float *freq_s,
                                              const float *dfreq_s, __bf16 *act, int tid) {
@@ -97,8 +105,8 @@ __device__ __forceinline__ void fourier_rows(const FusedArgs &a, int64_t tile, c
       for (int c = 0; c < kMaxIn; ++c) ph += fr[c] * xin[c];
       ph -= rintf(ph);
       const float s = __builtin_amdgcn_sinf(ph), c = __builtin_amdgcn_cosf(ph);
-      sn[jj] = (__bf16)s;
-      cs[jj] = (__bf16)c;
+      sn[jj] = (__bf16)(s * kNegLog2e);
+      cs[jj] = (__bf16)(c * kNegLog2e);
       if constexpr (NT > 0) {
         const f32x4_t dq = *reinterpret_cast<const f32x4_t *>(dfreq_s + f * kMaxIn);
 #pragma unroll
@@ -131,7 +139,7 @@ __global__ void __launch_bounds__(kThreads, 2) fused_mlp_kernel(FusedArgs a) {
   static_assert(RT % (1 + NT) == 0, "row tiles hold whole seed groups");
   __shared__ __attribute__((aligned(16))) __bf16 act[kRows * W];
   __shared__ __attribute__((aligned(16))) float freq_s[kNFreq * kMaxIn];   // B * scale, zero-padded
-  __shared__ __attribute__((aligned(16))) float dfreq_s[kNFreq * kMaxIn];  // 2 pi B * scale
+  __shared__ __attribute__((aligned(16))) float dfreq_s[kNFreq * kMaxIn];  // 2 pi B * scale * -log2(e)
   __shared__ float red[kWaves][kRows][4];                                  // output partials per wave
 
   const int tid = threadIdx.x, lane = tid & 63;
@@ -146,8 +154,10 @@ __global__ void __launch_bounds__(kThreads, 2) fused_mlp_kernel(FusedArgs a) {
       sc = a.scale[c];
     }
     freq_s[tid] = fv * sc;
-    dfreq_s[tid] = 6.283185307179586f * fv * sc;
+    dfreq_s[tid] = 6.283185307179586f * fv * sc * kNegLog2e;
   }
+  float tscale = a.out_scale;  // c^depth undoes the tangents' scale
+  for (int L = 0; L < a.depth; ++L) tscale *= kNegLn2;
   const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void *)a.W, 0, a.w_bytes, 0x00020000);
   bf16x8_t ring[kTilesPerWave];
 #pragma unroll
@@ -207,7 +217,7 @@ __global__ void __launch_bounds__(kThreads, 2) fused_mlp_kernel(FusedArgs a) {
             float v = red[0][n][o];
 #pragma unroll
             for (int w = 1; w < kWaves; ++w) v += red[w][n][o];
-            a.Y[((int64_t)a2 * a.M + seed) * O + o] = (a2 == 0 ? v + a.bout[o] : v) * a.out_scale;
+            a.Y[((int64_t)a2 * a.M + seed) * O + o] = a2 == 0 ? (v + a.bout[o]) * a.out_scale : v * tscale;
           }
         }
       }

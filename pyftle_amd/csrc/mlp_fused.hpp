@@ -134,9 +134,19 @@ __device__ __forceinline__ void layer_mma(__amdgpu_buffer_rsrc_t w, int lbytes, 
   }
 }
 
-// Hidden layer: h = SiLU(z + b) and, for the tangent rows, t * SiLU'(z), stored over the
-// layer's inputs in LDS (every wave has finished reading them).  Lane (r, g) holds
-// features 16 ft + 4 g + e of row 16 t + r for its tiles ft.
+// Inference stores its activations scaled by -log2(e).  The accumulators then hold
+// -log2(e) (z - b), so z' = -log2(e) z costs one fma with the bias, sigmoid(z) =
+// 1 / (1 + 2^z') needs no multiply, and h' = z' sigmoid(z) = -log2(e) SiLU(z) feeds the next
+// layer's unscaled weights consistently (the layer-0 inputs, the Fourier features, are stored
+// scaled alike).  SiLU'(z) = s + h (1 - s) is applied up to the constant c = -ln 2, as
+// d / c = h' + s (-log2(e) - h') (two operations instead of four): every hidden layer thus
+// scales the tangent rows by a further 1 / c, and the output undoes the product, c^depth.
+constexpr float kNegLog2e = -1.4426950408889634f;
+constexpr float kNegLn2 = -0.6931471805599453f;
+
+// Hidden layer: h' and, for the tangent rows, t (d / c), stored as bf16 over the layer's
+// inputs in LDS (every wave has finished reading them).  Lane (r, g) holds features
+// 16 ft + 4 g + e of row 16 t + r for its tiles ft.
 template <int W, int RT, int NT>
 __device__ __forceinline__ void hidden_epilogue(const f32x4_t (&acc)[Shape<W, RT>::kTilesPerWave][RT],
                                                 const float *bias_l, __bf16 *act, int ft0, int lane_in) {
@@ -155,13 +165,14 @@ __device__ __forceinline__ void hidden_epilogue(const f32x4_t (&acc)[Shape<W, RT
       bf16x4_t h, tg[NT > 0 ? NT : 1];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const float z = acc[i][b0][e] + bv[e];
-        const float s = sigmoid_f(z);
-        h[e] = (__bf16)(z * s);
+        const float z = fmaf(bv[e], kNegLog2e, acc[i][b0][e]);
+        const float s = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z));
+        const float hp = z * s;
+        h[e] = (__bf16)hp;
         if constexpr (NT > 0) {
-          const float d = s * (1.0f + z * (1.0f - s));
+          const float dd = fmaf(s, kNegLog2e - hp, hp);
 #pragma unroll
-          for (int a = 0; a < NT; ++a) tg[a][e] = (__bf16)(acc[i][b0 + 1 + a][e] * d);
+          for (int a = 0; a < NT; ++a) tg[a][e] = (__bf16)(acc[i][b0 + 1 + a][e] * dd);
         }
       }
       *reinterpret_cast<bf16x4_t *>(act + off + b0 * 16 * W) = h;
