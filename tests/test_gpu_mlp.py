@@ -272,6 +272,25 @@ def test_fused_persistent_tiles_match_layered_path(width, d_in, nt):
     assert _rel(Y[0][tail].cpu().numpy(), refs[0][tail].cpu().numpy()) < 2e-2
 
 
+@pytest.mark.parametrize("width,depth", [(256, 2), (256, 8), (512, 2), (512, 8)])
+def test_fused_depth_extremes_match_oracle(width, depth):
+    """The stored activations carry a factor -log2(e) and the tangent rows a further 1/ln 2
+    per hidden layer, undone at the output: the shallowest and deepest supported networks
+    still match the FP64 oracle (primal and every tangent)."""
+    from pyftle_amd.surrogate import FlowMapMLP, MLPSpec
+
+    d_in, d_out, nt = (3, 2, 2) if width == 256 else (4, 3, 3)
+    spec = MLPSpec(d_in=d_in, d_out=d_out, n_freq=64, width=width, depth=depth, sigma=0.7,
+                   scale=(0.5,) * (d_in - 1) + (0.1,), out_scale=1.3)
+    m = FlowMapMLP(spec, seed=40 + depth)
+    p = m.to_numpy()
+    x = np.random.default_rng(depth).uniform(0.0, 3.0, (333, d_in))
+    Y = m.forward_fused(torch.as_tensor(x, dtype=torch.float32, device="cuda"), n_tangents=nt).cpu().numpy()
+    assert _rel(Y[0], MO.forward(p, x)) < 3e-2
+    for a in range(nt):
+        assert _rel(Y[1 + a], MO.jvp(p, x, a)) < 3e-2
+
+
 def test_fused_rejects_unsupported_shapes():
     m = _small_model()  # width 128
     with pytest.raises(ValueError, match="fused inference"):
