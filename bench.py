@@ -1174,6 +1174,9 @@ def main():
     ap.add_argument("--tile-occupancy", action="store_true",
                     help="abc128 / gyre512: only measure how much of the window an LDS tile per work-group "
                          "could serve (share of particles inside their work-group's tile, by step)")
+    ap.add_argument("--issue-probe", action="store_true",
+                    help="only measure the issue cost (cycles) of MFMA, FP32 multiply, exp2 and reciprocal, "
+                         "alone and with an MFMA wave and a VALU wave sharing a SIMD")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="abc128: skip the surrogate leg (training and timing the 2D surrogate, ~1 min)")
@@ -1215,6 +1218,9 @@ def main():
 
     _native.load()
     torch.cuda.set_device(0)
+    if args.issue_probe:
+        print(json.dumps({"issue_probe": W.probe_issue_rates()}), flush=True)
+        return
     log(f"building workload {args.workload}")
     kw = {}
     if args.workload in ("surrogate2d", "surrogate3d") and args.surrogate_epochs > 0:

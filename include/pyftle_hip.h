@@ -267,12 +267,22 @@ int pyftle_advect_ftle(const pyftle_field *field, const pyftle_advect_opts *opts
  * pyftle_probe_load_rate: `blocks` work-groups of 256 lanes each issue `iters` (a
  * multiple of 4) 16-byte-per-lane wave loads, coalesced and L1-resident, from the first
  * 16 KiB of `buf`; one float per lane goes to `out` (blocks * 256).  Its rate is the peak
- * of the vector-memory path for such loads. */
+ * of the vector-memory path for such loads.
+ * pyftle_probe_issue_rates: the issue cost of the fused MLP kernels' instruction mix.
+ * `blocks` work-groups, each reserving 96 KB of LDS (one per CU), run `iters` iterations of
+ * 8 slots per wave: mode 0 one MFMA 16x16x32 bf16 per slot, 1 two FP32 multiplies, 2 two
+ * exp2, 3 two reciprocals, 4 one MFMA and two multiplies in the same wave (one wave per
+ * SIMD); modes 5, 6, 7 pair an MFMA wave (waves 0-3) with a multiply, exp2 or reciprocal
+ * wave (waves 4-7) on every SIMD.  cycles[2 b + k] (device uint64, 2 * blocks) receives
+ * the s_memtime cycles of wave 0 (k = 0) and wave 4 (k = 1, pairs only) of work-group b;
+ * out (blocks * 512 floats) keeps the results live. */
 int pyftle_advect_ftle_count_loads(const pyftle_field *field, const pyftle_advect_opts *opts,
                                    const void *positions_in, int32_t pos_dtype, int64_t n_groups,
                                    void *positions_out, double map_period, double *ftle_out,
                                    unsigned long long *loads, void *stream);
 int pyftle_probe_load_rate(const void *buf, int32_t blocks, int32_t iters, float *out, void *stream);
+int pyftle_probe_issue_rates(int32_t mode, int32_t blocks, int32_t iters, unsigned long long *cycles, float *out,
+                             void *stream);
 
 /* ---- FTLE tensors (FP64 in/out) ----
  * pyftle_flow_map_jacobian: cauchy_green.py:15-153.  positions / initial

@@ -118,6 +118,7 @@ SIGNATURES = {
     "pyftle_advect_ftle": [_FP, _OP, _vp, _i32, _i64, _vp, _f64, _vp, _vp],
     "pyftle_advect_ftle_count_loads": [_FP, _OP, _vp, _i32, _i64, _vp, _f64, _vp, _vp, _vp],
     "pyftle_probe_load_rate": [_vp, _i32, _i32, _vp, _vp],
+    "pyftle_probe_issue_rates": [_i32, _i32, _i32, _vp, _vp, _vp],
     "pyftle_flow_map_jacobian": [_vp, _vp, _i32, _i32, _i64, _vp, _vp],
     "pyftle_cauchy_green": [_vp, _i32, _i64, _vp, _vp],
     "pyftle_max_eigenvalue": [_vp, _i32, _i64, _vp, _vp],

@@ -161,6 +161,36 @@ def probe_load_rate(blocks: int = 2048, iters: int = 8192, repeats: int = 5, str
             "bytes_per_s": wave_loads * 1024 / best}
 
 
+ISSUE_MODES = {0: "mfma", 1: "mul", 2: "exp2", 3: "rcp", 4: "mfma+2mul one wave",
+               5: "mfma wave | mul wave", 6: "mfma wave | exp2 wave", 7: "mfma wave | rcp wave"}
+
+
+def probe_issue_rates(iters: int = 20000, stream=None) -> dict:
+    """Cycles per instruction of the fused MLP kernels' mix (``pyftle_probe_issue_rates``),
+    one work-group per CU: per mode, the mean s_memtime cycles per slot-instruction of the
+    MFMA wave (``first``) and, for the paired modes, of the VALU wave sharing its SIMD
+    (``second``).  A slot holds one MFMA or two VALU instructions."""
+    blocks = torch.cuda.get_device_properties(0).multi_processor_count
+    cycles = torch.zeros(2 * blocks, dtype=torch.int64, device="cuda")
+    out = torch.empty(blocks * 512, dtype=torch.float32, device="cuda")
+    st = stream or torch.cuda.current_stream()
+    res = {}
+    N.call("pyftle_probe_issue_rates", 0, blocks, 20 * iters, ptr(cycles), ptr(out), stream_handle(st))  # clock up
+    for mode, name in ISSUE_MODES.items():
+        N.call("pyftle_probe_issue_rates", mode, blocks, iters, ptr(cycles), ptr(out), stream_handle(st))
+        st.synchronize()
+        c = cycles.view(blocks, 2).double().mean(0).cpu().numpy() / iters
+        per_mfma = c[0] / 8
+        per_valu = (c[0] if mode in (1, 2, 3) else c[1]) / 16
+        row = {"cycles_per_iteration": [float(c[0]), float(c[1])]}
+        if mode in (0, 4, 5, 6, 7):
+            row["cycles_per_mfma"] = float(per_mfma)
+        if mode in (1, 2, 3, 5, 6, 7):
+            row["cycles_per_valu"] = float(per_valu)
+        res[name] = row
+    return res
+
+
 def advect(field: DeviceField, positions: torch.Tensor, *, method="rk4", time_mode="frozen",
            n_steps: int, h: float, first: int = 0, t_start: float = 0.0, vprev=None,
            have_prev=False, cfg: LaunchConfig | None = None, stream=None) -> torch.Tensor:
