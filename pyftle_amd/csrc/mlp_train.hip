@@ -404,15 +404,25 @@ struct ReduceArgs {
 
 // Gradients summed in a fixed order: the hidden weights over the GEMM splits (a thread per
 // weight), the rest over the row tiles (16 parameters per work-group, 16 interleaved partial
-// sums per parameter combined in order).
+// sums per parameter combined in order), the loss by one more work-group.
 __global__ void __launch_bounds__(256) train_reduce_kernel(ReduceArgs a) {
   const int W = a.W;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if ((int)blockIdx.x == (int)gridDim.x - 1) {
+    // the last work-group: the loss over the row tiles (thread j sums tiles j, j + 256, ...
+    // in order, then thread 0 the 256 partial sums in order), the batch offset and the step
+    __shared__ float lp[256];
     float l = 0.0f;
-    for (int k = 0; k < a.tiles; ++k) l += a.loss_part[k];
-    *a.loss += l;
-    *a.base_row += a.M;
-    if (a.step) *a.step += 1;
+    for (int k = threadIdx.x; k < a.tiles; k += blockDim.x) l += a.loss_part[k];
+    lp[threadIdx.x] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float t = 0.0f;
+      for (int j = 0; j < (int)blockDim.x; ++j) t += lp[j];
+      *a.loss += t;
+      *a.base_row += a.M;
+      if (a.step) *a.step += 1;
+    }
+    return;
   }
   if ((int)blockIdx.x < a.weight_blocks) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -589,9 +599,10 @@ WorkLayout work_layout(int64_t M, int W, int depth, int O, int splits, int rows_
   return w;
 }
 
-// rows per work-group: 64 (4 row tiles of 16) at both widths, so a 16384-row step spreads
-// over 256 work-groups
-int tile_rows(int) { return 64; }
+// rows per work-group: 32 at width 256, so that a 16384-row step spreads over 512 work-groups
+// and two share a CU (0.135 -> 0.121 ms per step; 16 rows: 0.156); 64 at width 512, where 32
+// rows spill at two work-groups per CU and run no faster (0.366 against 0.362 ms)
+int tile_rows(int w) { return w == 256 ? 32 : 64; }
 
 // forward layers then transposed layers 1..depth-1, bf16
 int packed_train_bytes(int W, int depth) { return 2 * (W * kFeat + 2 * (depth - 1) * W * W); }
@@ -653,8 +664,8 @@ int pyftle_mlp_train_grads(const pyftle_mlp_train_args *t, void *stream) {
   hipStream_t st = as_stream(stream);
   int rc;
   if (W == 256) {
-    rc = O == 1 ? launch_train<256, 4, 1>(a, st) : O == 2 ? launch_train<256, 4, 2>(a, st)
-                                                            : launch_train<256, 4, 3>(a, st);
+    rc = O == 1 ? launch_train<256, 2, 1>(a, st) : O == 2 ? launch_train<256, 2, 2>(a, st)
+                                                            : launch_train<256, 2, 3>(a, st);
   } else {
     rc = O == 1 ? launch_train<512, 4, 1>(a, st) : O == 2 ? launch_train<512, 4, 2>(a, st)
                                                             : launch_train<512, 4, 3>(a, st);
@@ -704,7 +715,7 @@ int pyftle_mlp_train_grads(const pyftle_mlp_train_args *t, void *stream) {
   r.base_row = t->base_row;
   r.step = t->step;
   r.M = t->M;
-  const int blocks = r.weight_blocks + (r.n_tiled + 15) / 16;
+  const int blocks = r.weight_blocks + (r.n_tiled + 15) / 16 + 1;  // + the loss's work-group
   train_reduce_kernel<<<blocks, 256, 0, st>>>(r);
   (void)n_params;
   return check_launch("mlp_train_grads reduce");
