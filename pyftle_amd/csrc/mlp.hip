@@ -195,27 +195,45 @@ struct GemmGroup {
   GemmArgs g[kGemmGroupMax];
   int start[kGemmGroupMax + 1];  // first work-group of each member
   int n;
+  int xcd;                       // deal the tiles of a split to one XCD (see gemm_group_kernel)
 };
 
+// Work-groups are dealt to the 8 XCDs round robin (work-group b to XCD b % 8), and each XCD
+// has its own L2.  With xcd set (every member's work-group count and start a multiple of 8,
+// its split count too), the tiles of one split -- which share their operand rows -- all go
+// to one XCD: work-group b of a member is XCD b % 8, slot b / 8 there, and takes split
+// (b % 8) + 8 (slot / tiles), tile slot % tiles.
 __global__ void __launch_bounds__(256) gemm_group_kernel(GemmGroup G) {
   const int b = blockIdx.x;
   int j = 0;
   while (j + 1 < G.n && b >= G.start[j + 1]) ++j;
   const GemmArgs &g = G.g[j];
-  const int nx = g.N / BN, ny = g.M / BM;
+  const int nx = g.N / BN, ny = g.M / BM, nt = nx * ny;
   const int loc = b - G.start[j];
-  gemm_tile<true, true, EPI_F32>(g, loc % nx, (loc / nx) % ny, loc / (nx * ny));
+  int tile, split;
+  if (G.xcd) {
+    const int slot = loc >> 3;
+    split = (loc & 7) + 8 * (slot / nt);
+    tile = slot % nt;
+  } else {
+    tile = loc % nt;
+    split = loc / nt;
+  }
+  gemm_tile<true, true, EPI_F32>(g, tile % nx, tile / nx, split);
 }
 
 int launch_gemm_group_f32(const GemmArgs *members, int n, hipStream_t st) {
   if (n < 1 || n > kGemmGroupMax) return fail("gemm group: 1..8 members");
   GemmGroup G{};
   G.n = n;
+  G.xcd = 1;
   int total = 0;
   for (int j = 0; j < n; ++j) {
     G.g[j] = members[j];
     G.start[j] = total;
-    total += (members[j].N / BN) * (members[j].M / BM) * (members[j].K / members[j].k_per_split);
+    const int splits = members[j].K / members[j].k_per_split;
+    if (splits % 8) G.xcd = 0;  // then also the starts stay multiples of 8
+    total += (members[j].N / BN) * (members[j].M / BM) * splits;
   }
   G.start[n] = total;
   gemm_group_kernel<<<total, 256, 0, st>>>(G);
