@@ -250,3 +250,18 @@ def test_probe_load_rate_is_positive():
     r = W.probe_load_rate(blocks=512, iters=1024, repeats=2)
     print(f"[probe] {r['wave_loads_per_s']:.3e} wave loads/s = {r['bytes_per_s'] / 1e12:.1f} TB/s", flush=True)
     assert r["wave_loads_per_s"] > 0
+
+
+def test_probe_issue_rates_cover_every_mode():
+    """The issue-rate probe runs all eight modes; an MFMA 16x16x32 bf16 cannot issue faster
+    than the 16 cycles its 16384 flops take at 1024 flops per cycle per SIMD."""
+    from pyftle_amd import window as W
+
+    r = W.probe_issue_rates(iters=500)
+    print(f"[probe] {r}", flush=True)
+    assert set(r) == set(W.ISSUE_MODES.values())
+    for name, row in r.items():
+        if "cycles_per_mfma" in row:
+            assert row["cycles_per_mfma"] >= 15.0, name
+        if "cycles_per_valu" in row:
+            assert row["cycles_per_valu"] > 0.0, name
