@@ -141,6 +141,8 @@ __device__ __forceinline__ void layer_mma(__amdgpu_buffer_rsrc_t w, int lbytes, 
 // scaled alike).  SiLU'(z) = s + h (1 - s) is applied up to the constant c = -ln 2, as
 // d / c = h' + s (-log2(e) - h') (two operations instead of four): every hidden layer thus
 // scales the tangent rows by a further 1 / c, and the output undoes the product, c^depth.
+// For z < -88, 2^z' overflows to +inf and the reciprocal gives s = 0, h' = 0, d = 0: the
+// limits.
 constexpr float kNegLog2e = -1.4426950408889634f;
 constexpr float kNegLn2 = -0.6931471805599453f;
 
