@@ -311,22 +311,37 @@ def surrogate2d_measure(torch, wl, args):
             f"disagreement p50/p90/p99 {qs}")
         del preds, sp
     # composite fields: RK4 wherever the disagreement exceeds tol (the fallback seeds of all
-    # window starts in one batched launch); times include the fallback
+    # window starts in one batched launch); times include the probe, the ensemble and the
+    # fallback.  The probe plan (the default) sends a tile to RK4 whole when more than half
+    # of its probed seeds fail.
     sweeps = {}
     for gate, jm, tols in (("ftle", M, (1e-3, 3e-3, 1e-2, 3e-2, 1e-1)),
                            ("displacement", 1, (1e-3, 1e-2, 3e-2))):
         sf = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], gate=gate, jacobian_members=jm)
+        share = sf.tile_fallback_max
         rows = []
         for tol in tols:
             sf.tol = tol
             res, ms = timed(lambda: sf.ftle_windows(seeds, t0s), 1)
             mean, worst = rmse_of([r.ftle for r in res])
             fb = sum(int(r.fallback.sum().item()) for r in res) / n_sw
+            skipped = sum(int(torch.isnan(r.spread).sum().item()) for r in res) / n_sw
             rows.append({"tol": tol, "ms": ms, "seeds_per_s": n_sw / (ms * 1e-3), "fallback_fraction": fb,
+                         "tile_rk4_fraction": skipped, "tile_fallback_max": share,
                          "rmse_mean": mean, "rmse_max": worst, "speedup_vs_gpu_rk4": rk4_ms / ms})
-            log(f"{gate} gate tol {tol:g}: {ms:.1f} ms, fallback {fb:.4f}, rmse/range {mean:.4f}")
+            log(f"{gate} gate tol {tol:g}: {ms:.1f} ms, fallback {fb:.4f} (tiles to RK4 {skipped:.4f}), "
+                f"rmse/range {mean:.4f}")
             del res
         sweeps[gate] = rows
+    # the specified setting without the probe plan (every seed through the ensemble)
+    sf = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], gate="ftle", tol=1e-3, probe=0)
+    res, ms = timed(lambda: sf.ftle_windows(seeds, t0s), 1)
+    mean, worst = rmse_of([r.ftle for r in res])
+    fb = sum(int(r.fallback.sum().item()) for r in res) / n_sw
+    unplanned = {"tol": 1e-3, "gate": "ftle", "ms": ms, "fallback_fraction": fb, "rmse_max": worst,
+                 "speedup_vs_gpu_rk4": rk4_ms / ms}
+    log(f"ftle gate tol 1e-3 without the probe plan: {ms:.1f} ms, fallback {fb:.4f}")
+    del res
     spec_row = next(r for r in sweeps["ftle"] if r["tol"] == 1e-3)
     ok = [r for g in sweeps.values() for r in g if r["rmse_max"] <= 0.02]
     best = max(ok, key=lambda r: r["speedup_vs_gpu_rk4"]) if ok else None
@@ -336,10 +351,11 @@ def surrogate2d_measure(torch, wl, args):
     a = alone["ftle"]
     return {
         "workload": "surrogate2d", "seed_windows": n_sw, "members": M, "network": "64 Fourier freqs, 5x256 SiLU",
-        "gate": "ftle: max over members |FTLE_member - FTLE_ensemble| / (FTLE range of the window)",
+        "gate": "ftle: max over members |FTLE_member - FTLE_ensemble| / (FTLE range of the window's probe)",
         "specified": {"tol": 1e-3, "gate": "ftle", "ms": spec_row["ms"], "fallback_fraction": spec_row["fallback_fraction"],
+                      "tile_rk4_fraction": spec_row["tile_rk4_fraction"],
                       "rmse_max": spec_row["rmse_max"], "speedup_vs_gpu_rk4": spec_row["speedup_vs_gpu_rk4"],
-                      "seeds_per_s": spec_row["seeds_per_s"]},
+                      "seeds_per_s": spec_row["seeds_per_s"], "without_probe_plan": unplanned},
         "best_within_2pct": None if best is None else dict(best, gate=best_gate),
         "surrogate_alone": alone, "tolerance_sweeps": sweeps,
         "mfma": {"achieved": a["tflops"], "peak": 2500.0, "unit": "TFLOP/s", "frac": a["tflops"] / 2500.0,
@@ -490,7 +506,7 @@ def run_surrogate3d(torch, wl, args):
     ref_node = sf.rk4_ftle(seeds, wl["t0s"][0])
     # the two rings start at different times, so the time weights round differently
     layout_diff = ((ref_cell - ref_node).abs().max() / (ref_node.max() - ref_node.min())).item()
-    del sf_cell, cell_field, ref_cell, ref_node
+    del sf_cell, ref_cell, ref_node
     torch.cuda.empty_cache()
     log(f"plain RK4, all {n_seeds} seeds, one window: {rk4_ms:.1f} ms with cell-pair records "
         f"({rk4_node_ms:.1f} ms node records; max FTLE difference / range {layout_diff:.1e})")
@@ -515,14 +531,26 @@ def run_surrogate3d(torch, wl, args):
         del f1
         log(f"t0 {t0}: " + ", ".join(f"{k} {v}" for k, v in row.items() if k != "t0"))
         per_t0.append(row)
-    # the specified setting end to end on every seed of one window: FTLE gate at 1e-3
-    sf_ftle.tol = 1e-3
-    res, sur_ms = timed(lambda: sf_ftle.ftle(seeds, wl["t0s"][0]))
-    fb = res.fallback.float().mean().item()
-    del res
     mean = lambda key, sub: sum(r[key][sub] for r in per_t0) / len(per_t0)  # noqa: E731
     fpq = S.mlp_flops_per_query(spec)
     jm4 = mean("ftle_gate", "ms")
+    # the specified setting end to end on every seed of one window: FTLE gate at 1e-3, the
+    # fallback on the cell-pair ring (the timed plain RK4's records); the probe plan (the
+    # default) sends a tile to RK4 whole when more than half of its probed seeds fail
+    spec_runs = {}
+    for name, probe in (("planned", 64), ("without_probe_plan", 0)):
+        sf_spec = SF.SurrogateFTLE(ens, cell_field, wl["T"], wl["h"], gate="ftle", tol=1e-3, chunk=1 << 19,
+                                   probe=probe)
+        share = sf_spec.tile_fallback_max
+        res, ms = timed(lambda: sf_spec.ftle(seeds, wl["t0s"][0]))
+        spec_runs[name] = {"ms": ms, "fallback_fraction": res.fallback.float().mean().item(),
+                           "tile_rk4_fraction": torch.isnan(res.spread).float().mean().item(),
+                           "speedup_vs_gpu_rk4": rk4_ms / ms}
+        log(f"specified setting ({name}): {ms:.1f} ms, fallback {spec_runs[name]['fallback_fraction']:.4f}")
+        del res
+    del cell_field, sf_spec
+    torch.cuda.empty_cache()
+    sur_ms, fb = spec_runs["planned"]["ms"], spec_runs["planned"]["fallback_fraction"]
     rows4 = n_seeds * wl["members"] * 4
     # estimated time at each tolerance: the gate's pass plus RK4 on the fallback fraction
     sweep = []
@@ -540,9 +568,11 @@ def run_surrogate3d(torch, wl, args):
         "warmup": 1, "ms_per_step": sur_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16", "data": "synthetic", "config": dict(wl["config"], fallback_tol=1e-3, gate="ftle"),
         "surrogate": {
-            "gate": "ftle: max over members |FTLE_member - FTLE_ensemble| / (FTLE range of the window)",
+            "gate": "ftle: max over members |FTLE_member - FTLE_ensemble| / (FTLE range of the window's probe)",
             "specified": {"tol": 1e-3, "ms": sur_ms, "fallback_fraction": fb,
-                          "speedup_vs_gpu_rk4": rk4_ms / sur_ms, "seeds_per_s": n_seeds / (sur_ms * 1e-3)},
+                          "tile_rk4_fraction": spec_runs["planned"]["tile_rk4_fraction"], "tile_fallback_max": share,
+                          "speedup_vs_gpu_rk4": rk4_ms / sur_ms, "seeds_per_s": n_seeds / (sur_ms * 1e-3),
+                          "without_probe_plan": spec_runs["without_probe_plan"]},
             "tolerance_sweep_held_out": sweep, "best_within_2pct": best,
             "ftle_rmse_over_range_mean_gate_pass": mean("ftle_gate", "rmse"),
             "ftle_rmse_over_range_mean_jac_1": mean("jac1", "rmse"),

@@ -467,3 +467,31 @@ def test_clough_tocher_factors_and_patch_restatement():
                + b4**3 * c0003)
         worst = max(worst, abs(val - ct(x[None])[0]))
     assert worst < 1e-13
+
+
+def test_surrogate_probe_tiles_cover_the_seeds():
+    """The probe plan's tiling (CPU tensors): tile ids in range, probe indices distinct, every
+    tile of a seed grid probed, and fewer tiles per axis when a tile would hold under 32
+    probed seeds."""
+    import torch
+
+    from pyftle_amd.surrogate_ftle import SurrogateFTLE
+
+    xs = (torch.arange(2048, dtype=torch.float64) + 0.5) / 1024.0
+    ys = (torch.arange(1024, dtype=torch.float64) + 0.5) / 1024.0
+    cy, cx = torch.meshgrid(ys, xs, indexing="ij")
+    seeds = torch.stack([cx.reshape(-1), cy.reshape(-1)], 1)
+    sf = SurrogateFTLE(None, None, 10.0, 0.05)
+    plan = sf._tiles_of(seeds)
+    tid, idx, n_tiles = plan.tile, plan.probe, plan.n_tiles
+    assert n_tiles == 256 and int(plan.tile_seeds.sum()) == seeds.shape[0] and int(tid.min()) == 0 and int(tid.max()) == 255
+    assert idx.numel() == seeds.shape[0] // 64 and torch.unique(idx).numel() == idx.numel()
+    per_tile = torch.bincount(tid[idx], minlength=n_tiles)
+    assert int(per_tile.min()) >= 100 and int(per_tile.max()) <= 160
+    # 256 x 256 seeds: 1024 probes, 5 tiles per axis (about 41 probes each)
+    small = seeds.reshape(1024, 2048, 2)[:256, :256].reshape(-1, 2).contiguous()
+    plan = sf._tiles_of(small)
+    tid, idx, n_tiles = plan.tile, plan.probe, plan.n_tiles
+    assert n_tiles == 25 and int(torch.bincount(tid[idx], minlength=25).min()) >= 20
+    with pytest.raises(ValueError):
+        SurrogateFTLE(None, None, 10.0, 0.05, tile_fallback_max=1.5)
