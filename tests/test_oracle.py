@@ -366,3 +366,88 @@ def test_window_linear_stretch_ftle_closed_form_and_mutants():
     # the unmutated restatement inside the helper agrees with the oracle
     fr, _ = _mutant_window_linear(vat, x0, t0, h, n, "none")
     np.testing.assert_allclose(fr, ftle, rtol=1e-12)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_c_port_clean_under_sanitizers(dim, tmp_path):
+    """The oracle's C restatement built with the address and undefined-behaviour sanitizers
+    (``oracle/sanitize_check.c`` driver, serial, -O1, contraction off): interpolation at
+    points inside, on the upper edges and outside the grid, and every integrator in frozen
+    and time-linear mode with particles that leave the grid, run with no sanitizer report
+    and bit-equal to the -O3 OpenMP library on the same inputs."""
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc absent")
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    exe = str(tmp_path / "sanitize_check")
+    build = subprocess.run(
+        ["gcc", "-O1", "-g", "-fno-omit-frame-pointer", "-ffp-contract=off", "-fno-fast-math",
+         "-fsanitize=address,undefined,float-cast-overflow", "-fno-sanitize-recover=all",
+         os.path.join(here, "cpu_path.c"), os.path.join(here, "sanitize_check.c"), "-lm", "-o", exe],
+        capture_output=True, text=True)
+    if build.returncode != 0 and "asan" in build.stderr.lower():
+        pytest.skip("sanitizer runtime unavailable: " + build.stderr[-200:])
+    assert build.returncode == 0, build.stderr
+    rng = np.random.default_rng(7 + dim)
+    shape = (17, 13) if dim == 2 else (9, 8, 7)
+    n_snap, n_steps, P, npts = 6, 5, 40, 200
+    grid = int(np.prod(shape))
+    mins = np.array([0.1, -0.2, 0.3][:dim] + [0.0] * (3 - dim))
+    deltas = np.array([0.05, 0.07, 0.06][:dim] + [1.0] * (3 - dim))
+    snaps = rng.standard_normal((n_snap, dim, grid))
+    ext = np.array(shape) - 1
+    # particles: most inside, some near and beyond the edges (they leave the grid)
+    x0 = mins[:dim] + deltas[:dim] * ext * rng.uniform(-0.1, 1.1, size=(P, dim))
+    # index-space points: inside, exactly on the upper edges, outside on both sides
+    pts = rng.uniform(-1.5, ext + 0.5, size=(npts, dim))
+    pts[:dim, :] = ext
+    pts[dim:2 * dim, :] = ext - 1
+    t0, h, snap_t0, snap_dt = 0.0, 0.1, 0.0, 0.1
+    inp = tmp_path / "in.bin"
+    with open(inp, "wb") as f:
+        np.array([dim, *shape, *([1] * (3 - dim)), n_snap, P, n_steps, npts], dtype=np.int64).tofile(f)
+        np.concatenate([mins, deltas, [snap_t0, snap_dt, t0, h]]).astype(np.float64).tofile(f)
+        snaps.astype(np.float64).tofile(f)
+        x0.astype(np.float64).tofile(f)
+        pts.astype(np.float64).tofile(f)
+    out = tmp_path / "out.bin"
+    run = subprocess.run([exe, str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    assert run.returncode == 0 and "runtime error" not in run.stderr and "Sanitizer" not in run.stderr, run.stderr
+    got = np.fromfile(out, dtype=np.float64)
+    # the same inputs through the -O3 OpenMP library
+    lib = ON.cpu_path()
+    dp = ON._dp
+    v0 = np.ascontiguousarray(snaps[0, 0])
+    ref_i = np.empty(npts)
+    pc = np.ascontiguousarray(pts)
+    if dim == 2:
+        lib.cpu_interp2d(dp(v0), shape[0], shape[1], dp(pc), npts, dp(ref_i))
+    else:
+        lib.cpu_interp3d(dp(v0), *shape, dp(pc), npts, dp(ref_i))
+    parts = [ref_i]
+    import ctypes
+
+    n = (ctypes.c_int * 3)(*(list(shape) + [1] * (3 - dim)))
+    comps = np.ascontiguousarray(snaps)
+    for method in range(3):
+        for mode in range(2):
+            x = np.array(x0, dtype=np.float64, copy=True, order="C")
+            assert lib.cpu_advect(dim, n, dp(mins), dp(deltas), dp(comps), n_snap, snap_t0, snap_dt, mode, dp(x), P,
+                                  t0, h, n_steps, method, 2) == 0
+            parts.append(x.reshape(-1))
+    ref = np.concatenate(parts)
+    assert got.shape == ref.shape
+    assert np.array_equal(got, ref)
+    assert np.any(ref_i == 0.0) and np.any(ref_i != 0.0)
+    # the sanitizers are live: one frozen step past the last snapshot reads beyond the buffer
+    hdr = np.fromfile(inp, dtype=np.int64, count=8)
+    hdr[6] = n_snap + 1
+    raw = bytearray(open(inp, "rb").read())
+    raw[:64] = hdr.tobytes()
+    bad_in = tmp_path / "bad.bin"
+    bad_in.write_bytes(bytes(raw))
+    run = subprocess.run([exe, str(bad_in), str(tmp_path / "bad_out.bin")], capture_output=True, text=True,
+                         timeout=120)
+    assert run.returncode != 0 and "heap-buffer-overflow" in run.stderr
