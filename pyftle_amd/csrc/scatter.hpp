@@ -146,6 +146,9 @@ __device__ __forceinline__ void ct_record(const char *slot, int64_t rec, double 
 // derivative along direction g (toward the neighbouring triangle's centroid,
 // g = -1/2 on the hull) is made linear along each edge, and C1 continuity inside
 // fixes the ordinates next to and at the centroid.  b: barycentric coordinates.
+// The ordinates (c3000 ... c0003) and the cross-edge terms are formed in the order of
+// SciPy's CloughTocher2DInterpolator evaluation (scipy/interpolate/interpnd.pyx,
+// BSD licence), so that results agree with it to rounding.
 __device__ __forceinline__ double ct_patch(const double *P, const double *fv, const double (*gr)[2],
                                            const double *g, const double *b) {
   const double e12x = P[2] - P[0], e12y = P[3] - P[1];
