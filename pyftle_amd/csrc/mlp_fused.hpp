@@ -91,16 +91,22 @@ __device__ __forceinline__ float sigmoid_f(float z) {
 }
 
 // acc[i][t] = (output features of this wave's tile i) x (rows of row tile t), over the K
-// inputs held in LDS.  On entry ring[i] holds tile i of k-step 0 of this layer; as soon as
-// the 4 MFMAs of a tile have been issued its slot is refilled with the same tile of the
-// next k-step (or of k-step 0 of the layer at `next`), so each load has a whole k-step
-// (the wave's other MFMAs, interleaved with the other work-group's wave on the SIMD) to land.
-template <int W, int RT, int K>
-__device__ __forceinline__ void layer_mma(__amdgpu_buffer_rsrc_t w, int lbytes, int next_lbytes,
-                                          const __bf16 *act, bf16x8_t (&ring)[Shape<W, RT>::kTilesPerWave],
-                                          f32x4_t (&acc)[Shape<W, RT>::kTilesPerWave][RT], int ft0, int lane_in) {
+// inputs held in LDS.  The weight fragments travel through a ring PD k-steps deep: on entry
+// ring[p][i] holds tile i of k-step p (p < PD) of this layer; as soon as the MFMAs of a tile
+// have been issued at k-step ks its slot ks % PD is refilled with the same tile of k-step
+// ks + PD (of this layer, or of the layer at `next`, whose k-steps continue the count), so
+// each load has PD k-steps of the wave's MFMAs (interleaved with the other waves on the SIMD)
+// to land.  Every layer's k-step count must be a multiple of PD, so the slot of a k-step is
+// the same whichever layer prefetched it.
+template <int W, int RT, int K, int PD>
+__device__ __forceinline__ void layer_mma_ring(__amdgpu_buffer_rsrc_t w, int lbytes, int next_lbytes,
+                                               const __bf16 *act,
+                                               bf16x8_t (&ring)[PD][Shape<W, RT>::kTilesPerWave],
+                                               f32x4_t (&acc)[Shape<W, RT>::kTilesPerWave][RT], int ft0,
+                                               int lane_in) {
   constexpr int kTilesPerWave = Shape<W, RT>::kTilesPerWave, kRowTiles = RT;
   constexpr int KS = K / 32;
+  static_assert(KS % PD == 0, "the ring depth divides the k-steps of every layer");
   const int lane = opaque_lane(lane_in);
   const int r = lane & 15, g = lane >> 4;
 #pragma unroll
@@ -123,15 +129,25 @@ __device__ __forceinline__ void layer_mma(__amdgpu_buffer_rsrc_t w, int lbytes, 
     for (int i = 0; i < kTilesPerWave; ++i) {
 #pragma unroll
       for (int t = 0; t < kRowTiles; ++t)
-        acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring[i], bfr[ks & 1][t], acc[i][t], 0, 0, 0);
-      const int sofs =
-          ks + 1 < KS ? lbytes + frag_bytes<W>(ks + 1, ft0 + i) : next_lbytes + frag_bytes<W>(0, ft0 + i);
-      ring[i] = load_frag(w, lane * 16, sofs);
+        acc[i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ring[ks % PD][i], bfr[ks & 1][t], acc[i][t], 0, 0, 0);
+      const int kn = ks + PD;
+      const int sofs = kn < KS ? lbytes + frag_bytes<W>(kn, ft0 + i) : next_lbytes + frag_bytes<W>(kn - KS, ft0 + i);
+      ring[ks % PD][i] = load_frag(w, lane * 16, sofs);
     }
     // keep the next k-step's loads in this one: left free, the scheduler sinks each load
     // to just before its use and the MFMAs wait on L2 latency
     __builtin_amdgcn_sched_barrier(0);
   }
+}
+
+// The ring one k-step deep (the inference kernel: its accumulators leave no room for more).
+template <int W, int RT, int K>
+__device__ __forceinline__ void layer_mma(__amdgpu_buffer_rsrc_t w, int lbytes, int next_lbytes,
+                                          const __bf16 *act, bf16x8_t (&ring)[Shape<W, RT>::kTilesPerWave],
+                                          f32x4_t (&acc)[Shape<W, RT>::kTilesPerWave][RT], int ft0, int lane_in) {
+  constexpr int kTilesPerWave = Shape<W, RT>::kTilesPerWave;
+  layer_mma_ring<W, RT, K, 1>(w, lbytes, next_lbytes, act,
+                              *reinterpret_cast<bf16x8_t(*)[1][kTilesPerWave]>(&ring), acc, ft0, lane_in);
 }
 
 // Inference stores its activations scaled by -log2(e).  The accumulators then hold

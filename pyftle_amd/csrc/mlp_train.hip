@@ -175,9 +175,15 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
     freq_s[tid] = fv * sc;
   }
   const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void *)a.Wp, 0, a.wp_bytes, 0x00020000);
-  bf16x8_t ring[TPW];
+  // weight fragments PD k-steps ahead: at width 256 a k-step is only 2 RT MFMAs per
+  // fragment, so one k-step of cover leaves the L2 latency exposed; width 512 has no
+  // registers left for more
+  constexpr int PD = W == 256 ? 4 : 1;
+  bf16x8_t ring[PD][TPW];
 #pragma unroll
-  for (int i = 0; i < TPW; ++i) ring[i] = load_frag(wr, lane * 16, frag_bytes<W>(0, ft0 + i));
+  for (int p = 0; p < PD; ++p)
+#pragma unroll
+    for (int i = 0; i < TPW; ++i) ring[p][i] = load_frag(wr, lane * 16, frag_bytes<W>(p, ft0 + i));
   __syncthreads();
 
   // Fourier features [sin | cos] of the tile's rows
@@ -206,7 +212,7 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
   f32x4_t acc[TPW][RT];
   // Forward.  Layer 0 (K = 128) and the last layer are peeled off the loop so that no
   // accumulator array is carried around a loop holding two instantiations of the MFMA loop.
-  layer_mma<W, RT, kFeat>(wr, layer_bytes<W>(0), layer_bytes<W>(1), act, ring, acc, ft0, lane);
+  layer_mma_ring<W, RT, kFeat, PD>(wr, layer_bytes<W>(0), layer_bytes<W>(1), act, ring, acc, ft0, lane);
   for (int L = 0;; ++L) {
     __syncthreads();  // every wave has read this layer's inputs (and their transposed copy is out)
     {
@@ -238,11 +244,11 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
     __syncthreads();
     transpose_out<W, RT>(act, W, a.HT + (int64_t)L * W * M, M, col0, tid);
     if (L + 2 == a.depth) break;
-    layer_mma<W, RT, W>(wr, layer_bytes<W>(L + 1), layer_bytes<W>(L + 2), act, ring, acc, ft0, lane);
+    layer_mma_ring<W, RT, W, PD>(wr, layer_bytes<W>(L + 1), layer_bytes<W>(L + 2), act, ring, acc, ft0, lane);
   }
   {
     const int L = a.depth - 1;
-    layer_mma<W, RT, W>(wr, layer_bytes<W>(L), tlayer_bytes<W>(L, a.depth), act, ring, acc, ft0, lane);
+    layer_mma_ring<W, RT, W, PD>(wr, layer_bytes<W>(L), tlayer_bytes<W>(L, a.depth), act, ring, acc, ft0, lane);
     const float *bl = a.params + b_off<W>(L);
     __syncthreads();  // every wave has read the last layer's inputs
     // output layer (FP32): y = SiLU(z + b) Wout^T + bout, then the MSE gradient
@@ -372,7 +378,7 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
     if (l == 0) break;
     const int lb = tlayer_bytes<W>(l, a.depth);
     const int nlb = l > 1 ? tlayer_bytes<W>(l - 1, a.depth) : layer_bytes<W>(0);
-    layer_mma<W, RT, W>(wr, lb, nlb, act, ring, acc, ft0, lane);  // dH of layer l - 1
+    layer_mma_ring<W, RT, W, PD>(wr, lb, nlb, act, ring, acc, ft0, lane);  // dH of layer l - 1
     __syncthreads();
     const __bf16 *dact = a.DACT + (((((int64_t)(l - 1) * ntiles + tile) * kWaves + wave) * TPW * RT * 64) * 4);
     const int ln = opaque_lane(lane);
