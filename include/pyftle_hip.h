@@ -350,6 +350,12 @@ int pyftle_batch_launch(pyftle_batch *batch, void *stream);
 int pyftle_batch_info(const pyftle_batch *batch, int64_t *n_items, int32_t *grid_blocks,
                       int32_t *block_threads);
 int pyftle_batch_destroy(pyftle_batch *batch);
+/* Instrumentation (no reference counterpart): pyftle_batch_count_loads runs the batch once
+ * (not through its graph) with a counting copy of the persistent kernel, for RK4 batches on
+ * CELL / CELL_PAIR records: the same results, and the wave-level 16-byte record loads the
+ * launch issues are added to *loads (device uint64, accumulated), as
+ * pyftle_advect_ftle_count_loads does for one window.  Run outside timed regions. */
+int pyftle_batch_count_loads(pyftle_batch *batch, unsigned long long *loads, void *stream);
 
 /* ---- batched windows without device allocation ----
  * No reference counterpart (pyFTLE allocates nothing on a GPU).  For callers

@@ -61,6 +61,7 @@ _SIGS = {
     "pyftle_batch_info": [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
                           ctypes.POINTER(ctypes.c_int32)],
     "pyftle_batch_destroy": [ctypes.c_void_p],
+    "pyftle_batch_count_loads": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "pyftle_batch_run_noalloc": [ctypes.POINTER(BatchField), ctypes.c_int32, ctypes.POINTER(BatchOpts),
                                  ctypes.c_void_p],
 }
@@ -177,6 +178,14 @@ class BatchPlan:
     def run(self, stream=None) -> list[torch.Tensor]:
         N.call("pyftle_batch_launch", self._h, stream_handle(stream))
         return [w.ftle for w in self.windows]
+
+    def count_loads(self, stream=None) -> int:
+        """Runs the batch once with a counting copy of the persistent kernel (RK4 on
+        per-cell records; the same FTLE fields) and returns the wave-level 16-byte record
+        loads it issued (``pyftle_batch_count_loads``).  Waits for the device."""
+        loads = torch.zeros(1, dtype=torch.int64, device=self.windows[0].positions.device)
+        N.call("pyftle_batch_count_loads", self._h, loads.data_ptr(), stream_handle(stream))
+        return int(loads.item())
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -75,6 +75,38 @@ __global__ void __launch_bounds__(512) batch_kernel_prefix(const AdvectArgs *__r
   }
 }
 
+// Counting copy of batch_kernel (RK4 windows on per-cell records): the same items, the
+// same per-item device function and results, and every wave adds the 16-byte record loads
+// it issues to *loads.  Run outside timed regions.
+template <int DIM, int TMODE, bool PAIR>
+__global__ void __launch_bounds__(512) batch_count_kernel(const AdvectArgs *__restrict__ table,
+                                                          const int2 *__restrict__ items, int n_items,
+                                                          unsigned int *counter, unsigned long long *loads) {
+  __shared__ int s_item;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
+    if (it >= n_items) break;
+    const int f = __builtin_amdgcn_readfirstlane(items[it].x);
+    const int b = __builtin_amdgcn_readfirstlane(items[it].y);
+    AdvectArgs a = table[f];
+    a.load_count = loads;
+    advect_block_cell<DIM, PYFTLE_RK4, TMODE, PAIR, true, true, true>(a, b);
+  }
+}
+
+template <int DIM>
+static const void *count_kernel_for(int tmode, int pair) {
+  if (tmode == PYFTLE_TIME_LINEAR) return reinterpret_cast<const void *>(&batch_count_kernel<DIM, PYFTLE_TIME_LINEAR, true>);
+  if (tmode == PYFTLE_TIME_FROZEN)
+    return pair ? reinterpret_cast<const void *>(&batch_count_kernel<DIM, PYFTLE_TIME_FROZEN, true>)
+                : reinterpret_cast<const void *>(&batch_count_kernel<DIM, PYFTLE_TIME_FROZEN, false>);
+  return pair ? reinterpret_cast<const void *>(&batch_count_kernel<DIM, PYFTLE_TIME_STEADY, true>)
+              : reinterpret_cast<const void *>(&batch_count_kernel<DIM, PYFTLE_TIME_STEADY, false>);
+}
+
 using BatchFn = void (*)(const AdvectArgs *, const int2 *, int, unsigned int *);
 
 template <int DIM, int METHOD, int TMODE, bool PAIR>
@@ -360,6 +392,25 @@ int pyftle_batch_launch(pyftle_batch *b, void *stream) {
     return PYFTLE_OK;
   }
   return enqueue(b, st);
+}
+
+int pyftle_batch_count_loads(pyftle_batch *b, unsigned long long *loads, void *stream) {
+  if (!b) return fail("batch_count_loads: NULL batch");
+  if (!loads) return fail("batch_count_loads: loads is NULL");
+  if (!b->cell || b->method != PYFTLE_RK4)
+    return fail("batch_count_loads: counts RK4 batches on CELL / CELL_PAIR records");
+  if (b->n_items == 0) return PYFTLE_OK;
+  hipStream_t st = as_stream(stream);
+  reset_counter_kernel<<<1, 1, 0, st>>>(b->d_counter);
+  const AdvectArgs *table = b->d_table;
+  const int2 *items = b->d_items;
+  int n = (int)b->n_items;
+  unsigned int *counter = b->d_counter;
+  void *args[] = {(void *)&table, (void *)&items, (void *)&n, (void *)&counter, (void *)&loads};
+  const void *k = b->dim == 2 ? count_kernel_for<2>(b->tmode, b->pair) : count_kernel_for<3>(b->tmode, b->pair);
+  hipError_t e = hipLaunchKernel(k, dim3(b->grid), dim3(b->threads), args, 0, st);
+  if (e != hipSuccess) return fail(std::string("batch_count_loads: ") + hipGetErrorString(e), PYFTLE_EHIP);
+  return check_launch("batch_count_loads");
 }
 
 int pyftle_batch_info(const pyftle_batch *b, int64_t *n_items, int32_t *grid_blocks,

@@ -163,3 +163,42 @@ def test_batch_rejects_malformed_windows():
             BatchPlan([b], method="rk4", time_mode="linear")
     with pytest.raises(ValueError):
         BatchPlan([], method="rk4")
+
+
+def test_batch_counting_launch_equals_single_window_counts():
+    """The counting copy of the persistent kernel (pyftle_batch_count_loads) leaves every
+    window's FTLE bit-identical, and the record loads it counts equal the sum of the counts of
+    the single-window kernel over the same windows at the same block shape (a work item is
+    one block of one window, so its waves are the single-window kernel's waves)."""
+    from pyftle_amd import _native as N
+    from pyftle_amd import window as W
+    from pyftle_amd.batch import BatchConfig, BatchPlan, Window
+    from pyftle_amd.field import DeviceField, GridGeometry
+
+    wins = []
+    for (nx, ny), n_snap, (sx, sy), steps, t0 in (((64, 32), 13, (40, 20), 20, 0.1), ((48, 24), 21, (33, 5), 35, 0.2),
+                                                  ((96, 48), 11, (50, 25), 16, 0.0)):
+        vels, pts = H.double_gyre_window(nx=nx, ny=ny, n_snap=n_snap)
+        f = DeviceField(GridGeometry.from_reference((nx, ny), pts), dtype="float32", layout=N.LAYOUT_CELL_PAIR,
+                        ring_slots=n_snap, n_snapshots=n_snap, t0=0.0, dt_snap=0.1)
+        for k, v in enumerate(vels):
+            f.pack(k, v)
+        pos = torch.as_tensor(H.seeds_2d(sx, sy), device="cuda")
+        wins.append(Window(field=f, positions=pos, n_steps=steps, h=0.05, t_start=t0, map_period=steps * 0.05))
+    plan = BatchPlan(wins, method="rk4", time_mode="linear", cfg=BatchConfig(groups_per_block=32, use_graph=1))
+    plan.run()
+    torch.cuda.synchronize()
+    before = [w.ftle.clone() for w in wins]
+    loads = plan.count_loads()
+    assert all(torch.equal(b, w.ftle) for b, w in zip(before, wins))
+    total = 0
+    for w in wins:
+        cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+        single = W.advect_ftle(w.field, w.positions, method="rk4", time_mode="linear", n_steps=w.n_steps, h=w.h,
+                               t_start=w.t_start, map_period=w.map_period, count_loads=cnt,
+                               cfg=W.LaunchConfig(groups_per_block=32, xcd_remap=0))
+        assert torch.equal(single, w.ftle)
+        total += int(cnt.item())
+    print(f"[gpu] batch counted {loads} record wave loads, single windows {total}", flush=True)
+    assert loads == total > 0
+    plan.close()
