@@ -483,6 +483,14 @@ int pyftle_mlp_train_adam(float *params, float *m, float *v, const float *grads,
                           int32_t d_out, float beta1, float beta2, float eps, float weight_decay, float lr_max,
                           float lr_min, int32_t total_steps, int32_t adam_step0, const int32_t *step,
                           void *packed, void *stream);
+/* A whole step: pyftle_mlp_train_grads followed by pyftle_mlp_train_adam (with
+ * packed = args->W_packed, params = args->params, step = args->step, which must not be NULL),
+ * with the Adam update applied by the gradient reduction itself: three launches instead of
+ * four, the same parameters, moments, packed weights, gradients, loss, batch offset and step
+ * index bit for bit.  args->params and args->W_packed are updated in place. */
+int pyftle_mlp_train_step(const pyftle_mlp_train_args *args, float *m, float *v, float beta1, float beta2,
+                          float eps, float weight_decay, float lr_max, float lr_min, int32_t total_steps,
+                          int32_t adam_step0, void *stream);
 
 #ifdef __cplusplus
 }

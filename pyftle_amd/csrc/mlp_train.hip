@@ -45,6 +45,7 @@ struct TrainArgs {
   float loss_scale;        // 1 / (M * O)
   __bf16 *FT, *HT, *DZT, *DACT;
   float *bias_part, *wout_part, *bout_part, *loss_part;
+  int *step;               // step index, += 1 here (nothing reads it before the reduction)
 };
 
 // flat parameter layout: per layer W_l (W x fan_l) then b_l (W); then Wout (O x W), bout (O)
@@ -165,6 +166,7 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
   const int64_t grow0 = *a.base_row + (int64_t)tile * kRows;  // data row of the tile's row 0
   const int64_t col0 = (int64_t)tile * kRows;                 // its column in the step's buffers
   const int64_t M = a.M;
+  if (a.step && tile == 0 && tid == 0) *a.step += 1;
   {
     const int q = tid >> 2, c = tid & 3;
     float fv = 0.0f, sc = 0.0f;
@@ -393,6 +395,61 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
   }
 }
 
+// Adam on the FP32 master weights with the cosine learning-rate schedule evaluated from
+// *step (the step's index; bias corrections for Adam step t0 + *step + 1).  Shared by
+// train_adam_kernel and the reduction's fused update, so both give the same bits.
+struct AdamParams {
+  float *w, *m, *v;
+  __bf16 *packed;  // forward and transposed bf16 packing of the hidden weights
+  float b1, b2, eps, wd, lr_max, lr_min;
+  int total, t0;
+  const int *step;
+};
+
+struct AdamSchedule {
+  float lr, bc1, bc2;
+};
+
+__device__ __forceinline__ AdamSchedule adam_schedule(const AdamParams &p) {
+  const int k = *p.step;
+  const double c = cos(3.141592653589793 * (double)k / (double)(p.total > 0 ? p.total : 1));
+  AdamSchedule s;
+  s.lr = (float)((double)p.lr_min + 0.5 * ((double)p.lr_max - (double)p.lr_min) * (1.0 + c));
+  s.bc1 = 1.0f - powf(p.b1, (float)(p.t0 + k + 1));
+  s.bc2 = 1.0f - powf(p.b2, (float)(p.t0 + k + 1));
+  return s;
+}
+
+// updates parameter i with gradient g; returns the new weight
+__device__ __forceinline__ float adam_apply(const AdamParams &p, const AdamSchedule &s, int64_t i, float g) {
+  const float gi = g + p.wd * p.w[i];
+  const float mi = p.b1 * p.m[i] + (1.0f - p.b1) * gi;
+  const float vi = p.b2 * p.v[i] + (1.0f - p.b2) * gi * gi;
+  p.m[i] = mi;
+  p.v[i] = vi;
+  const float wi = p.w[i] - s.lr * (mi / s.bc1) / (sqrtf(vi / s.bc2) + p.eps);
+  p.w[i] = wi;
+  return wi;
+}
+
+// index of element (m, k) of a rows x K matrix in the fragment-packed order of
+// pyftle_mlp_pack_fused
+__device__ __forceinline__ int64_t packed_index(int m, int k, int rows) {
+  const int ft = m >> 4, ks = k >> 5, ln = (m & 15) + 16 * ((k & 31) >> 3);
+  return (((int64_t)ks * (rows >> 4) + ft) * 64 + ln) * 8 + (k & 7);
+}
+
+// the bf16 packings of hidden weight (o, i) of layer l (fan inputs), whose forward packing
+// starts at element pk: forward order, and for l > 0 the transposed layer the backward pass reads
+__device__ __forceinline__ void pack_weight(__bf16 *packed, int W, int depth, int l, int fan, int64_t pk, int o,
+                                            int i, float wi) {
+  packed[pk + packed_index(o, i, W)] = (__bf16)wi;
+  if (l > 0) {  // W_l^T: fan x W, element (i, o)
+    const int64_t tb = (int64_t)W * kFeat + (int64_t)(depth - 1) * W * W + (int64_t)(l - 1) * W * W;
+    packed[tb + packed_index(i, o, fan)] = (__bf16)wi;
+  }
+}
+
 struct ReduceArgs {
   const float *part;       // weight-gradient partials, layer after layer: (splits, W, fan_l)
   int splits;
@@ -404,13 +461,16 @@ struct ReduceArgs {
   float *grads;
   float *loss;             // += the step's loss
   int64_t *base_row;       // += M
-  int *step;               // += 1 (may be NULL)
   int M;
+  AdamParams adam;         // ADAM: the update applied to each summed gradient
 };
 
 // Gradients summed in a fixed order: the hidden weights over the GEMM splits (a thread per
 // weight), the rest over the row tiles (16 parameters per work-group, 16 interleaved partial
-// sums per parameter combined in order), the loss by one more work-group.
+// sums per parameter combined in order), the loss by one more work-group.  With ADAM each
+// summed gradient is also applied to its parameter at once (the step's Adam update, as
+// train_adam_kernel would apply it) and the hidden weights' bf16 packings are rewritten.
+template <bool ADAM>
 __global__ void __launch_bounds__(256) train_reduce_kernel(ReduceArgs a) {
   const int W = a.W;
   if ((int)blockIdx.x == (int)gridDim.x - 1) {
@@ -426,7 +486,6 @@ __global__ void __launch_bounds__(256) train_reduce_kernel(ReduceArgs a) {
       for (int j = 0; j < (int)blockDim.x; ++j) t += lp[j];
       *a.loss += t;
       *a.base_row += a.M;
-      if (a.step) *a.step += 1;
     }
     return;
   }
@@ -449,7 +508,12 @@ __global__ void __launch_bounds__(256) train_reduce_kernel(ReduceArgs a) {
           s3 += src[(int64_t)(s + 3) * nw];
         }
         for (; s < a.splits; ++s) s0 += src[(int64_t)s * nw];
-        a.grads[off + loc] = (s0 + s1) + (s2 + s3);
+        const float gsum = (s0 + s1) + (s2 + s3);
+        a.grads[off + loc] = gsum;
+        if constexpr (ADAM) {
+          const float wi = adam_apply(a.adam, adam_schedule(a.adam), off + loc, gsum);
+          pack_weight(a.adam.packed, W, a.depth, l, fan, woff, (int)(loc / fan), (int)(loc % fan), wi);
+        }
         return;
       }
       off += nw + W;
@@ -501,14 +565,8 @@ __global__ void __launch_bounds__(256) train_reduce_kernel(ReduceArgs a) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) t += ps[j][qi];
     a.grads[dst] = t;
+    if constexpr (ADAM) adam_apply(a.adam, adam_schedule(a.adam), dst, t);
   }
-}
-
-// index of element (m, k) of a rows x K matrix in the fragment-packed order of
-// pyftle_mlp_pack_fused
-__device__ __forceinline__ int64_t packed_index(int m, int k, int rows) {
-  const int ft = m >> 4, ks = k >> 5, ln = (m & 15) + 16 * ((k & 31) >> 3);
-  return (((int64_t)ks * (rows >> 4) + ft) * 64 + ln) * 8 + (k & 7);
 }
 
 __global__ void __launch_bounds__(256) pack_train_kernel(const float *params, int W, int depth, __bf16 *out) {
@@ -519,13 +577,7 @@ __global__ void __launch_bounds__(256) pack_train_kernel(const float *params, in
     const int64_t nw = (int64_t)W * fan;
     if (q < pk + nw) {
       const int64_t loc = q - pk;
-      const int o = (int)(loc / fan), i = (int)(loc % fan);
-      const __bf16 v = (__bf16)params[off + loc];
-      out[pk + packed_index(o, i, W)] = v;
-      if (l > 0) {  // W_l^T: fan x W, element (i, o)
-        const int64_t tb = (int64_t)W * kFeat + (int64_t)(depth - 1) * W * W + (int64_t)(l - 1) * W * W;
-        out[tb + packed_index(i, o, fan)] = v;
-      }
+      pack_weight(out, W, depth, l, fan, pk, (int)(loc / fan), (int)(loc % fan), params[off + loc]);
       return;
     }
     off += nw + W;
@@ -533,39 +585,20 @@ __global__ void __launch_bounds__(256) pack_train_kernel(const float *params, in
   }
 }
 
-// Adam on the FP32 master weights with the cosine learning-rate schedule evaluated from
-// *step (the step's index, advanced by the reduction kernel; bias corrections for Adam step
-// t0 + *step + 1), writing the hidden weights' forward and transposed bf16 packings.
-__global__ void __launch_bounds__(256) train_adam_kernel(float *w, float *m, float *v, const float *g, int64_t n,
-                                                         int W, int depth, float b1, float b2, float eps, float wd,
-                                                         float lr_max, float lr_min, int total, int t0,
-                                                         const int *step, __bf16 *packed) {
+// Adam over every parameter (gradients from train_reduce_kernel<false>), writing the hidden
+// weights' forward and transposed bf16 packings.
+__global__ void __launch_bounds__(256) train_adam_kernel(AdamParams ad, const float *g, int64_t n, int W,
+                                                         int depth) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const int k = *step;
-  const double c = cos(3.141592653589793 * (double)k / (double)(total > 0 ? total : 1));
-  const float lr = (float)((double)lr_min + 0.5 * ((double)lr_max - (double)lr_min) * (1.0 + c));
-  const float bc1 = 1.0f - powf(b1, (float)(t0 + k + 1));
-  const float bc2 = 1.0f - powf(b2, (float)(t0 + k + 1));
-  const float gi = g[p] + wd * w[p];
-  const float mi = b1 * m[p] + (1.0f - b1) * gi;
-  const float vi = b2 * v[p] + (1.0f - b2) * gi * gi;
-  m[p] = mi;
-  v[p] = vi;
-  const float wi = w[p] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
-  w[p] = wi;
+  const float wi = adam_apply(ad, adam_schedule(ad), p, g[p]);
   int64_t off = 0, pk = 0;
   for (int l = 0; l < depth; ++l) {
     const int fan = l == 0 ? kFeat : W;
     const int64_t nw = (int64_t)W * fan;
     if (p < off + nw) {
       const int64_t loc = p - off;
-      const int o = (int)(loc / fan), i = (int)(loc % fan);
-      packed[pk + packed_index(o, i, W)] = (__bf16)wi;
-      if (l > 0) {
-        const int64_t tb = (int64_t)W * kFeat + (int64_t)(depth - 1) * W * W + (int64_t)(l - 1) * W * W;
-        packed[tb + packed_index(i, o, fan)] = (__bf16)wi;
-      }
+      pack_weight(ad.packed, W, depth, l, fan, pk, (int)(loc / fan), (int)(loc % fan), wi);
       return;
     }
     if (p < off + nw + W) return;  // a bias
@@ -627,7 +660,48 @@ int64_t pyftle_mlp_train_work_bytes(int32_t M, int32_t width, int32_t depth, int
   return work_layout(M, width, depth, O, splits, tile_rows(width)).total;
 }
 
+namespace pyftle {
+namespace {
+
+// the three launches of a step: train_kernel, the grouped weight-gradient GEMM and the
+// reduction (with the Adam update when `adam` is given)
+int train_step_launches(const pyftle_mlp_train_args *t, const AdamParams *adam, hipStream_t st);
+
+}  // namespace
+}  // namespace pyftle
+
 int pyftle_mlp_train_grads(const pyftle_mlp_train_args *t, void *stream) {
+  return train_step_launches(t, nullptr, as_stream(stream));
+}
+
+int pyftle_mlp_train_step(const pyftle_mlp_train_args *t, float *m, float *v, float beta1, float beta2, float eps,
+                          float weight_decay, float lr_max, float lr_min, int32_t total_steps, int32_t adam_step0,
+                          void *stream) {
+  if (!t) return fail("mlp_train_step: NULL arguments");
+  if (!m || !v || !t->step) return fail("mlp_train_step: NULL Adam moments or step index");
+  AdamParams ad{};
+  ad.w = const_cast<float *>(t->params);
+  ad.m = m;
+  ad.v = v;
+  ad.packed = static_cast<__bf16 *>(const_cast<void *>(t->W_packed));
+  ad.b1 = beta1;
+  ad.b2 = beta2;
+  ad.eps = eps;
+  ad.wd = weight_decay;
+  ad.lr_max = lr_max;
+  ad.lr_min = lr_min;
+  ad.total = total_steps;
+  ad.t0 = adam_step0;
+  ad.step = t->step;
+  return train_step_launches(t, &ad, as_stream(stream));
+}
+
+}  // extern "C"
+
+namespace pyftle {
+namespace {
+
+int train_step_launches(const pyftle_mlp_train_args *t, const AdamParams *adam, hipStream_t st) {
   if (!t) return fail("mlp_train_grads: NULL arguments");
   if (!t->x || !t->targets || !t->base_row || !t->scale || !t->Bf || !t->W_packed || !t->params || !t->grads ||
       !t->loss || !t->work)
@@ -667,7 +741,7 @@ int pyftle_mlp_train_grads(const pyftle_mlp_train_args *t, void *stream) {
   a.wout_part = reinterpret_cast<float *>(wk + wl.wout);
   a.bout_part = reinterpret_cast<float *>(wk + wl.bout);
   a.loss_part = reinterpret_cast<float *>(wk + wl.loss);
-  hipStream_t st = as_stream(stream);
+  a.step = t->step;
   int rc;
   if (W == 256) {
     rc = O == 1 ? launch_train<256, 2, 1>(a, st) : O == 2 ? launch_train<256, 2, 2>(a, st)
@@ -719,13 +793,22 @@ int pyftle_mlp_train_grads(const pyftle_mlp_train_args *t, void *stream) {
   r.grads = t->grads;
   r.loss = t->loss;
   r.base_row = t->base_row;
-  r.step = t->step;
   r.M = t->M;
   const int blocks = r.weight_blocks + (r.n_tiled + 15) / 16 + 1;  // + the loss's work-group
-  train_reduce_kernel<<<blocks, 256, 0, st>>>(r);
+  if (adam) {
+    r.adam = *adam;
+    train_reduce_kernel<true><<<blocks, 256, 0, st>>>(r);
+  } else {
+    train_reduce_kernel<false><<<blocks, 256, 0, st>>>(r);
+  }
   (void)n_params;
   return check_launch("mlp_train_grads reduce");
 }
+
+}  // namespace
+}  // namespace pyftle
+
+extern "C" {
 
 int pyftle_mlp_train_adam(float *params, float *m, float *v, const float *grads, int32_t width, int32_t depth,
                           int32_t d_out, float beta1, float beta2, float eps, float weight_decay, float lr_max,
@@ -737,9 +820,21 @@ int pyftle_mlp_train_adam(float *params, float *m, float *v, const float *grads,
   int64_t n = 0;
   for (int l = 0; l < depth; ++l) n += (int64_t)width * (l == 0 ? mlpf::kFeat : width) + width;
   n += (int64_t)d_out * width + d_out;
-  train_adam_kernel<<<(unsigned)((n + 255) / 256), 256, 0, as_stream(stream)>>>(
-      params, m, v, grads, n, width, depth, beta1, beta2, eps, weight_decay, lr_max, lr_min, total_steps,
-      adam_step0, step, static_cast<__bf16 *>(packed));
+  AdamParams ad{};
+  ad.w = params;
+  ad.m = m;
+  ad.v = v;
+  ad.packed = static_cast<__bf16 *>(packed);
+  ad.b1 = beta1;
+  ad.b2 = beta2;
+  ad.eps = eps;
+  ad.wd = weight_decay;
+  ad.lr_max = lr_max;
+  ad.lr_min = lr_min;
+  ad.total = total_steps;
+  ad.t0 = adam_step0;
+  ad.step = step;
+  train_adam_kernel<<<(unsigned)((n + 255) / 256), 256, 0, as_stream(stream)>>>(ad, grads, n, width, depth);
   return check_launch("mlp_train_adam");
 }
 

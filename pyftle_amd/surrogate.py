@@ -47,6 +47,7 @@ _SIGS = {
                                  _i32, _vp, _vp],
     "pyftle_mlp_train_work_bytes": [_i32, _i32, _i32, _i32, _i32],
     "pyftle_mlp_train_grads": [_vp, _vp],
+    "pyftle_mlp_train_step": [_vp, _vp, _vp, _f32, _f32, _f32, _f32, _f32, _f32, _i32, _i32, _vp],
     "pyftle_mlp_pack_train": [_vp, _i32, _i32, _vp, _vp],
     "pyftle_mlp_train_adam": [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _f32, _f32, _f32, _f32, _f32, _f32, _i32,
                               _i32, _vp, _vp, _vp],
@@ -372,7 +373,8 @@ class FusedTrainStep:
     ``*base .. *base + batch - 1`` of (X, T / out_scale) (``pyftle_mlp_train_grads``, which
     also adds the loss to ``loss``, advances ``base`` and, given, the step index ``step``),
     then ``pyftle_mlp_train_adam`` (Adam at that step's learning rate, writing the packed
-    bf16 weights the next step reads)."""
+    bf16 weights the next step reads); ``step`` runs both with the update applied by the
+    gradient reduction (``pyftle_mlp_train_step``)."""
 
     def __init__(self, model: FlowMapMLP, X: torch.Tensor, Tn: torch.Tensor, batch: int, base: torch.Tensor,
                  loss: torch.Tensor, splits: int = 0, step: torch.Tensor | None = None):
@@ -410,6 +412,16 @@ class FusedTrainStep:
 
     def grads(self):
         N.call("pyftle_mlp_train_grads", ctypes.byref(self.args), stream_handle())
+
+    def step(self, lr_max: float, lr_min: float, total: int, step0: int):
+        """``grads()`` then ``adam()`` with the update applied by the gradient reduction
+        (``pyftle_mlp_train_step``: three launches instead of four, the same bits); needs the
+        step index tensor given at construction."""
+        if self.args.step is None:
+            raise ValueError("the fused step needs the device step index")
+        m, s = self.model, self.model.spec
+        N.call("pyftle_mlp_train_step", ctypes.byref(self.args), _p(m.m), _p(m.v), 0.9, 0.999, 1e-8, 0.0,
+               float(lr_max), float(lr_min), int(total), int(step0), stream_handle())
 
     def adam(self, lr_max: float, lr_min: float, total: int, step0: int, step: torch.Tensor):
         m, s = self.model, self.model.spec
@@ -467,8 +479,7 @@ def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20,
 
     def body():
         if use_fused:
-            fstep.grads()  # also advances base and the step index
-            fstep.adam(lr, lr_min, total, step0, k_dev)
+            fstep.step(lr, lr_min, total, step0)  # also advances base and the step index
             return
         idx = base + ar
         torch.index_select(Xs, 0, idx, out=xb)

@@ -378,3 +378,37 @@ def test_fused_training_reduces_loss_like_layered():
     print(f"[gpu] fused loss {out[True][0]:.3e} -> {out[True][-1]:.3e}; layered {out[False][-1]:.3e}", flush=True)
     assert out[True][-1] < 0.1 * out[True][0]
     assert out[True][-1] < 2.0 * out[False][-1]
+
+
+@pytest.mark.parametrize("width,depth,d_in,d_out", [(256, 5, 3, 2), (512, 3, 4, 3)])
+def test_fused_train_step_equals_grads_then_adam(width, depth, d_in, d_out):
+    """pyftle_mlp_train_step (the Adam update applied by the gradient reduction) against
+    pyftle_mlp_train_grads followed by pyftle_mlp_train_adam, three steps from the same state:
+    parameters, moments, packed weights, gradients, loss, batch offset and step index bit for bit."""
+    from pyftle_amd.surrogate import FusedTrainStep
+
+    rng = np.random.default_rng(5)
+    M = 2048
+    lo, hi = [0, 0, 0, 0][:d_in], [2, 1, 10, 3][:d_in]
+    X = torch.as_tensor(rng.uniform(lo, hi, (4 * M, d_in)), dtype=torch.float32, device="cuda")
+    Tn = torch.as_tensor(rng.standard_normal((4 * M, d_out)) * 0.3, dtype=torch.float32, device="cuda")
+    state = []
+    for fused in (False, True):
+        m = _train_model(width, depth, d_in, d_out, seed=2)
+        base = torch.zeros(1, dtype=torch.int64, device="cuda")
+        loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+        k = torch.full((1,), -1, dtype=torch.int32, device="cuda")
+        st = FusedTrainStep(m, X, Tn, M, base, loss, splits=4, step=k)
+        for _ in range(3):
+            if fused:
+                st.step(3e-3, 3e-5, 10, 0)
+            else:
+                st.grads()
+                st.adam(3e-3, 3e-5, 10, 0, k)
+        torch.cuda.synchronize()
+        state.append((m.params.clone(), m.m.clone(), m.v.clone(), m.grads.clone(), st.packed.clone(),
+                      loss.clone(), base.clone(), k.clone()))
+    names = ("params", "m", "v", "grads", "packed", "loss", "base", "step")
+    for name, a, b in zip(names, *state):
+        assert torch.equal(a, b), name
+    assert int(state[1][7].item()) == 2 and int(state[1][6].item()) == 3 * M
