@@ -871,12 +871,15 @@ def run_mlp_train(torch, wl, args):
     m = S.FlowMapMLP(spec, seed=1)
     epochs = max(2, args.steps)
     fused = not args.train_layered
-    S.train(m, wl["X"], wl["D"], epochs=1, batch=batch, lr=3e-3, graph=args.train_graph, fused=fused)  # warm-up
+    sp = args.train_splits
+    S.train(m, wl["X"], wl["D"], epochs=1, batch=batch, lr=3e-3, graph=args.train_graph, fused=fused,
+            splits=sp)  # warm-up
     torch.cuda.synchronize()
     a = torch.cuda.Event(enable_timing=True)
     b = torch.cuda.Event(enable_timing=True)
     a.record()
-    losses = S.train(m, wl["X"], wl["D"], epochs=epochs, batch=batch, lr=3e-3, graph=args.train_graph, fused=fused)
+    losses = S.train(m, wl["X"], wl["D"], epochs=epochs, batch=batch, lr=3e-3, graph=args.train_graph, fused=fused,
+                     splits=sp)
     b.record()
     torch.cuda.synchronize()
     ms = a.elapsed_time(b)
@@ -887,7 +890,8 @@ def run_mlp_train(torch, wl, args):
     print(json.dumps({"metric": "surrogate training TFLOP/s", "value": tf, "unit": "TFLOP/s", "n_gpus": 1,
                       "steps": steps, "ms_per_step": ms / steps, "higher_is_better": True, "dtype": "bf16",
                       "data": "synthetic", "config": dict(wl["config"], graph=bool(args.train_graph),
-                                                          kernels="layered" if args.train_layered else "fused"),
+                                                          kernels="layered" if args.train_layered else "fused",
+                                                          splits=sp or "default"),
                       "mfma": {"achieved": tf, "peak": 2500.0, "unit": "TFLOP/s", "frac": tf / 2500.0,
                                "flops_per_row": S.mlp_train_flops_per_row(spec)},
                       "final_loss": losses[-1]}), flush=True)
@@ -1226,6 +1230,8 @@ def main():
     ap.add_argument("--train-graph", type=int, default=0, help="mlp_train: replay each step as a HIP graph")
     ap.add_argument("--train-shape", default="2d", choices=["2d", "3d"], help="mlp_train: the 2D or the 3D network")
     ap.add_argument("--train-layered", action="store_true", help="mlp_train: the layer-by-layer kernels")
+    ap.add_argument("--train-splits", type=int, default=0,
+                    help="mlp_train: split-K of the fused step's weight-gradient GEMMs (0 = its default)")
     ap.add_argument("--tile-occupancy", action="store_true",
                     help="abc128 / gyre512: only measure how much of the window an LDS tile per work-group "
                          "could serve (share of particles inside their work-group's tile, by step)")

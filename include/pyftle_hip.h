@@ -453,7 +453,7 @@ int pyftle_mlp_fused_forward(const float *x, int32_t d_in, int32_t M, const floa
  * transposed layers 1..depth-1.  Writes every parameter's MSE gradient (mean over the
  * M x d_out outputs) to grads, adds the loss to *loss and advances *base_row by M, all on
  * the stream, so a step has no host-side arguments that change.  M a multiple of 64 and
- * of 64 * splits (split-K of the weight-gradient GEMMs); 64 Fourier frequencies, width
+ * of 128 * splits (split-K of the weight-gradient GEMMs); 64 Fourier frequencies, width
  * 256 or 512, depth 2..8, d_in <= 4, d_out <= 3; work of pyftle_mlp_train_work_bytes,
  * 256-byte aligned.  Gradients are summed in a fixed order (deterministic). */
 typedef struct pyftle_mlp_train_args {

@@ -42,20 +42,25 @@ __device__ __forceinline__ float dsilu_f(float z) {
 }
 
 constexpr int BM = 128, BN = 128, BK = 64, LDSK = BK + 8;  // padded LDS rows (bank spread)
+// The training step's weight-gradient group stages K 128 deep: twice the MFMAs per barrier
+// and per global-load round trip (its work-groups have a CU each and hide nothing else).
+constexpr int BK_GROUP = 128;
 
 // A 128 x 64 (rows x k) operand tile held in registers between its global
 // load and its LDS store, so the next tile's loads overlap the MFMAs of the
 // current one.  KC: the source is k-contiguous (element (r,k) at
 // src[r*ld + k]); otherwise r-contiguous (element (r,k) at src[k*ld + r]).
-template <bool KC>
+template <bool KC, int BKT = BK>
 struct TileLoader {
-  bf16x8 v[4];
+  static constexpr int kReps = 128 * BKT / 8 / 256;  // 16-byte chunks per thread
+  static constexpr int kChunksPerRow = BKT / 8;
+  bf16x8 v[kReps];
   __device__ __forceinline__ void load(const __bf16 *src, int64_t ld, int r0, int k0, int tid) {
 #pragma unroll
-    for (int rep = 0; rep < 4; ++rep) {
-      const int c = tid + 256 * rep;  // 1024 chunks of 8 elements
+    for (int rep = 0; rep < kReps; ++rep) {
+      const int c = tid + 256 * rep;  // 128 * BKT / 8 chunks of 8 elements
       if (KC) {
-        const int row = c >> 3, kq = (c & 7) * 8;
+        const int row = c / kChunksPerRow, kq = (c % kChunksPerRow) * 8;
         v[rep] = *reinterpret_cast<const bf16x8 *>(src + (int64_t)(r0 + row) * ld + k0 + kq);
       } else {
         const int k = c >> 4, rq = (c & 15) * 8;
@@ -63,12 +68,12 @@ struct TileLoader {
       }
     }
   }
-  __device__ __forceinline__ void store(__bf16 (*sm)[LDSK], int tid) const {
+  __device__ __forceinline__ void store(__bf16 (*sm)[BKT + 8], int tid) const {
 #pragma unroll
-    for (int rep = 0; rep < 4; ++rep) {
+    for (int rep = 0; rep < kReps; ++rep) {
       const int c = tid + 256 * rep;
       if (KC) {
-        const int row = c >> 3, kq = (c & 7) * 8;
+        const int row = c / kChunksPerRow, kq = (c % kChunksPerRow) * 8;
         *reinterpret_cast<bf16x8 *>(&sm[row][kq]) = v[rep];
       } else {
         const int k = c >> 4, rq = (c & 15) * 8;
@@ -79,12 +84,13 @@ struct TileLoader {
   }
 };
 
-template <bool A_KC, bool B_KC, int EPI>
+template <bool A_KC, bool B_KC, int EPI, int BKT = BK>
 __device__ __forceinline__ void gemm_tile(const GemmArgs &g, const int bx, const int by, const int bz) {
   // One LDS buffer: operand tiles in the main loop, the output tile after it.
-  __shared__ __attribute__((aligned(16))) __bf16 smem[(BM + BN) * LDSK];
-  __bf16(*As)[LDSK] = reinterpret_cast<__bf16(*)[LDSK]>(smem);
-  __bf16(*Bs)[LDSK] = reinterpret_cast<__bf16(*)[LDSK]>(smem + BM * LDSK);
+  constexpr int LK = BKT + 8;  // padded LDS rows (bank spread)
+  __shared__ __attribute__((aligned(16))) __bf16 smem[(BM + BN) * LK];
+  __bf16(*As)[LK] = reinterpret_cast<__bf16(*)[LK]>(smem);
+  __bf16(*Bs)[LK] = reinterpret_cast<__bf16(*)[LK]>(smem + BM * LK);
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -100,21 +106,21 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, const int bx, const
       for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
 
   const int lr = lane & 31, lk = (lane >> 5) * 8;
-  TileLoader<A_KC> la;
-  TileLoader<B_KC> lb;
+  TileLoader<A_KC, BKT> la;
+  TileLoader<B_KC, BKT> lb;
   la.load(g.A, g.lda, m0, kb, tid);
   lb.load(g.B, g.ldb, n0, kb, tid);
   la.store(As, tid);
   lb.store(Bs, tid);
   __syncthreads();
-  for (int k0 = kb; k0 < ke; k0 += BK) {
-    const bool more = k0 + BK < ke;
+  for (int k0 = kb; k0 < ke; k0 += BKT) {
+    const bool more = k0 + BKT < ke;
     if (more) {
-      la.load(g.A, g.lda, m0, k0 + BK, tid);
-      lb.load(g.B, g.ldb, n0, k0 + BK, tid);
+      la.load(g.A, g.lda, m0, k0 + BKT, tid);
+      lb.load(g.B, g.ldb, n0, k0 + BKT, tid);
     }
 #pragma unroll
-    for (int ks = 0; ks < BK; ks += 16) {
+    for (int ks = 0; ks < BKT; ks += 16) {
       bf16x8 a[2], b[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8 *>(&As[wm * 64 + i * 32 + lr][ks + lk]);
@@ -147,7 +153,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, const int bx, const
   } else {
     // Stage the bf16 tile in LDS (the operand buffers are free after the
     // main loop), then write 16-byte chunks per lane, coalesced along rows.
-    static_assert(BM * (BN + 8) <= (BM + BN) * LDSK, "the output tile fits in the operand buffer");
+    static_assert(BM * (BN + 8) <= (BM + BN) * LK, "the output tile fits in the operand buffer");
     __bf16(*Ct)[BN + 8] = reinterpret_cast<__bf16(*)[BN + 8]>(smem);  // EPI_BIAS_SILU: Z, then SiLU(Z)
 #pragma unroll
     for (int pass = 0; pass < (EPI == EPI_BIAS_SILU ? 2 : 1); ++pass) {
@@ -203,7 +209,7 @@ struct GemmGroup {
 // its split count too), the tiles of one split -- which share their operand rows -- all go
 // to one XCD: work-group b of a member is XCD b % 8, slot b / 8 there, and takes split
 // (b % 8) + 8 (slot / tiles), tile slot % tiles.
-__global__ void __launch_bounds__(256) gemm_group_kernel(GemmGroup G) {
+__global__ void __launch_bounds__(256, 2) gemm_group_kernel(GemmGroup G) {
   const int b = blockIdx.x;
   int j = 0;
   while (j + 1 < G.n && b >= G.start[j + 1]) ++j;
@@ -219,7 +225,7 @@ __global__ void __launch_bounds__(256) gemm_group_kernel(GemmGroup G) {
     tile = loc % nt;
     split = loc / nt;
   }
-  gemm_tile<true, true, EPI_F32>(g, tile % nx, tile / nx, split);
+  gemm_tile<true, true, EPI_F32, BK_GROUP>(g, tile % nx, tile / nx, split);
 }
 
 int launch_gemm_group_f32(const GemmArgs *members, int n, hipStream_t st) {
@@ -229,6 +235,8 @@ int launch_gemm_group_f32(const GemmArgs *members, int n, hipStream_t st) {
   G.xcd = 1;
   int total = 0;
   for (int j = 0; j < n; ++j) {
+    if (members[j].k_per_split % BK_GROUP || members[j].K % members[j].k_per_split)
+      return fail("gemm group: each split must cover a multiple of 128 of K");
     G.g[j] = members[j];
     G.start[j] = total;
     const int splits = members[j].K / members[j].k_per_split;

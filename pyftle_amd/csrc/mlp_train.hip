@@ -713,8 +713,8 @@ int train_step_launches(const pyftle_mlp_train_args *t, const AdamParams *adam, 
     return fail("mlp_train_grads: 1 <= d_in <= 4 and 1 <= d_out <= 3");
   const int R = tile_rows(t->width);
   const int S = t->splits;
-  if (t->M <= 0 || t->M % R || S < 1 || t->M % (64 * S))
-    return fail("mlp_train_grads: M must be a multiple of " + std::to_string(R) + " and of 64 * splits");
+  if (t->M <= 0 || t->M % R || S < 1 || t->M % (128 * S))
+    return fail("mlp_train_grads: M must be a multiple of " + std::to_string(R) + " and of 128 * splits");
   if ((uintptr_t)t->params % 16 || (uintptr_t)t->W_packed % 16 || (uintptr_t)t->work % 256)
     return fail("mlp_train_grads: params and W_packed 16-byte, work 256-byte aligned");
   const int W = t->width, depth = t->depth, O = t->d_out;

@@ -365,7 +365,7 @@ class FlowMapMLP:
 def fused_train_supported(spec: MLPSpec, batch: int) -> bool:
     """Whether ``pyftle_mlp_train_grads`` covers this network and minibatch."""
     return (spec.n_freq == 64 and spec.width in (256, 512) and 2 <= spec.depth <= 8 and spec.d_in <= 4
-            and spec.d_out <= 3 and batch % 64 == 0)
+            and spec.d_out <= 3 and batch % 128 == 0)
 
 
 class FusedTrainStep:
@@ -382,9 +382,9 @@ class FusedTrainStep:
         s = model.spec
         if not fused_train_supported(s, batch):
             raise ValueError(f"the fused training step does not cover {s} with batch {batch}")
-        if splits <= 0:
-            splits = max(1, min(16, batch // 1024))
-        while batch % (64 * splits):
+        if splits <= 0:  # measured best at 16384 rows: 16 at width 256, 8 at width 512
+            splits = max(1, min(16 if s.width == 256 else 8, batch // 1024))
+        while batch % (128 * splits):
             splits -= 1
         self.model, self.batch, self.splits = model, batch, splits
         dev = X.device
@@ -432,7 +432,7 @@ class FusedTrainStep:
 
 def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20, batch: int = 32768,
           lr: float = 2e-3, lr_min: float = 2e-5, seed: int = 0, log=None, graph: bool = False,
-          fused: bool = True):
+          fused: bool = True, splits: int = 0):
     """Adam with a cosine learning-rate schedule over minibatches; returns per-epoch losses.
 
     Each epoch gathers a shuffled copy of the data once; a step takes the next ``batch``
@@ -444,7 +444,7 @@ def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20,
     layer-by-layer kernels (~45 launches).
     With ``graph`` the step is captured in a HIP graph after two eager steps and replayed;
     eager and replayed steps run the same kernels on the same buffers (bit-identical
-    weights).
+    weights).  ``splits``: split-K of the fused step's weight-gradient GEMMs (0: its default).
     """
     n = X.shape[0]
     if n < batch:
@@ -468,7 +468,7 @@ def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20,
         # is advanced by the gradient call before Adam reads it
         Tn = (T / model.spec.out_scale).contiguous()
         k_dev.fill_(-1)
-        fstep = FusedTrainStep(model, Xs, Ts, batch, base, acc, step=k_dev)
+        fstep = FusedTrainStep(model, Xs, Ts, batch, base, acc, splits=splits, step=k_dev)
     else:
         Tn = T
 
