@@ -200,9 +200,13 @@ class SurrogateFTLE:
         n, dim = seeds.shape
         m = max(1, n // self.probe)
         per_axis = max(1, min(self.tiles, int((m / 32.0) ** (1.0 / dim))))
-        lo = seeds.min(0).values
-        ext = (seeds.max(0).values - lo).clamp_min(1e-300)
-        cell = ((seeds - lo) / ext * per_axis).floor().clamp_(0, per_axis - 1).long()
+        fin = torch.isfinite(seeds)  # non-finite coordinates fall in tile 0 of their axis
+        lo = torch.where(fin, seeds, float("inf")).min(0).values
+        hi = torch.where(fin, seeds, float("-inf")).max(0).values
+        lo = torch.where(torch.isfinite(lo), lo, torch.zeros_like(lo))
+        ext = (hi - lo).nan_to_num(1.0, 1.0, 1.0).clamp_min(1e-300)
+        pos = ((seeds - lo) / ext * per_axis).nan_to_num(0.0, 0.0, 0.0)
+        cell = pos.floor().clamp_(0, per_axis - 1).long()
         tid = torch.zeros(n, dtype=torch.long, device=seeds.device)
         for a in range(dim):
             tid = tid * per_axis + cell[:, a]

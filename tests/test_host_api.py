@@ -493,5 +493,12 @@ def test_surrogate_probe_tiles_cover_the_seeds():
     plan = sf._tiles_of(small)
     tid, idx, n_tiles = plan.tile, plan.probe, plan.n_tiles
     assert n_tiles == 25 and int(torch.bincount(tid[idx], minlength=25).min()) >= 20
+    # non-finite coordinates do not break the tiling (they go to tile 0 of their axis)
+    bad = small.clone()
+    bad[5, 0] = float("nan")
+    bad[9, 1] = float("inf")
+    plan = sf._tiles_of(bad)
+    assert int(plan.tile.min()) >= 0 and int(plan.tile.max()) < plan.n_tiles
+    assert torch.equal(plan.tile[10:], sf._tiles_of(small).tile[10:])
     with pytest.raises(ValueError):
         SurrogateFTLE(None, None, 10.0, 0.05, tile_fallback_max=1.5)
