@@ -506,7 +506,7 @@ def run_surrogate3d(torch, wl, args):
     ref_node = sf.rk4_ftle(seeds, wl["t0s"][0])
     # the two rings start at different times, so the time weights round differently
     layout_diff = ((ref_cell - ref_node).abs().max() / (ref_node.max() - ref_node.min())).item()
-    del sf_cell, ref_cell, ref_node
+    del sf_cell, ref_node
     torch.cuda.empty_cache()
     log(f"plain RK4, all {n_seeds} seeds, one window: {rk4_ms:.1f} ms with cell-pair records "
         f"({rk4_node_ms:.1f} ms node records; max FTLE difference / range {layout_diff:.1e})")
@@ -548,7 +548,19 @@ def run_surrogate3d(torch, wl, args):
                            "speedup_vs_gpu_rk4": rk4_ms / ms}
         log(f"specified setting ({name}): {ms:.1f} ms, fallback {spec_runs[name]['fallback_fraction']:.4f}")
         del res
-    del cell_field, sf_spec
+    # looser gates end to end on every seed of the same window, judged by the full RK4 field
+    measured = []
+    for tol in (3e-2, 1e-1):
+        sf_spec = SF.SurrogateFTLE(ens, cell_field, wl["T"], wl["h"], gate="ftle", tol=tol, chunk=1 << 19)
+        res, ms = timed(lambda: sf_spec.ftle(seeds, wl["t0s"][0]))
+        row = {"tol": tol, "t0": wl["t0s"][0], "ms": ms, "fallback_fraction": res.fallback.float().mean().item(),
+               "tile_rk4_fraction": torch.isnan(res.spread).float().mean().item(),
+               "rmse": SF.relative_rmse(res.ftle, ref_cell), "speedup_vs_gpu_rk4": rk4_ms / ms}
+        measured.append(row)
+        log(f"FTLE gate tol {tol:g} on all seeds: {ms:.1f} ms, fallback {row['fallback_fraction']:.4f}, "
+            f"rmse/range {row['rmse']:.4f} against the full RK4 field, {row['speedup_vs_gpu_rk4']:.2f}x")
+        del res
+    del cell_field, sf_spec, ref_cell
     torch.cuda.empty_cache()
     sur_ms, fb = spec_runs["planned"]["ms"], spec_runs["planned"]["fallback_fraction"]
     rows4 = n_seeds * wl["members"] * 4
@@ -574,6 +586,7 @@ def run_surrogate3d(torch, wl, args):
                           "speedup_vs_gpu_rk4": rk4_ms / sur_ms, "seeds_per_s": n_seeds / (sur_ms * 1e-3),
                           "without_probe_plan": spec_runs["without_probe_plan"]},
             "tolerance_sweep_held_out": sweep, "best_within_2pct": best,
+            "measured_all_seeds_first_window": measured,
             "ftle_rmse_over_range_mean_gate_pass": mean("ftle_gate", "rmse"),
             "ftle_rmse_over_range_mean_jac_1": mean("jac1", "rmse"),
             "ms_surrogate_only_gate_pass": jm4, "ms_surrogate_only_jac_1": mean("jac1", "ms"),
