@@ -316,7 +316,7 @@ def surrogate2d_measure(torch, wl, args):
     # of its probed seeds fail.
     sweeps = {}
     for gate, jm, tols in (("ftle", M, (1e-3, 3e-3, 1e-2, 3e-2, 1e-1)),
-                           ("displacement", 1, (1e-3, 1e-2, 3e-2))):
+                           ("displacement", 1, (1e-3, 1e-2, 3e-2, 1e-1))):
         sf = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], gate=gate, jacobian_members=jm)
         share = sf.tile_fallback_max
         rows = []
@@ -548,16 +548,24 @@ def run_surrogate3d(torch, wl, args):
                            "speedup_vs_gpu_rk4": rk4_ms / ms}
         log(f"specified setting ({name}): {ms:.1f} ms, fallback {spec_runs[name]['fallback_fraction']:.4f}")
         del res
-    # looser gates end to end on every seed of the same window, judged by the full RK4 field
+    # looser gates end to end on every seed of the same window, judged by the full RK4 field: the
+    # FTLE gate (every member's Jacobian) and the displacement gate (one Jacobian member, domain units)
+    _, dsp = sf_disp.predict(seeds, wl["t0s"][0])
+    sub = dsp[torch.randperm(dsp.numel(), device=dev)[:1 << 20]].float()
+    disp_q = torch.quantile(sub, torch.tensor([0.5, 0.9, 0.99], device=dev)).tolist()
+    del dsp, sub
+    log(f"displacement disagreement p50/p90/p99 {disp_q}")
     measured = []
-    for tol in (3e-2, 1e-1):
-        sf_spec = SF.SurrogateFTLE(ens, cell_field, wl["T"], wl["h"], gate="ftle", tol=tol, chunk=1 << 19)
+    for gate, tol in (("ftle", 3e-2), ("ftle", 1e-1), ("displacement", 0.1), ("displacement", 0.3), ("displacement", 1.0)):
+        sf_spec = SF.SurrogateFTLE(ens, cell_field, wl["T"], wl["h"], gate=gate, tol=tol, chunk=1 << 19,
+                                   jacobian_members=1 if gate == "displacement" else None)
         res, ms = timed(lambda: sf_spec.ftle(seeds, wl["t0s"][0]))
-        row = {"tol": tol, "t0": wl["t0s"][0], "ms": ms, "fallback_fraction": res.fallback.float().mean().item(),
+        row = {"gate": gate, "tol": tol, "t0": wl["t0s"][0], "ms": ms,
+               "fallback_fraction": res.fallback.float().mean().item(),
                "tile_rk4_fraction": torch.isnan(res.spread).float().mean().item(),
                "rmse": SF.relative_rmse(res.ftle, ref_cell), "speedup_vs_gpu_rk4": rk4_ms / ms}
         measured.append(row)
-        log(f"FTLE gate tol {tol:g} on all seeds: {ms:.1f} ms, fallback {row['fallback_fraction']:.4f}, "
+        log(f"{gate} gate tol {tol:g} on all seeds: {ms:.1f} ms, fallback {row['fallback_fraction']:.4f}, "
             f"rmse/range {row['rmse']:.4f} against the full RK4 field, {row['speedup_vs_gpu_rk4']:.2f}x")
         del res
     del cell_field, sf_spec, ref_cell
@@ -572,9 +580,9 @@ def run_surrogate3d(torch, wl, args):
         sweep.append({"tol": tol, "fallback_fraction": f, "rmse_mean": mean(f"tol{tol:g}", "rmse"),
                       "rmse_max": max(r[f"tol{tol:g}"]["rmse"] for r in per_t0), "ms_estimate": est,
                       "speedup_vs_gpu_rk4_estimate": rk4_ms / est})
-    ok = [r for r in sweep if r["rmse_max"] <= 0.02]
-    best = max(ok, key=lambda r: r["speedup_vs_gpu_rk4_estimate"]) if ok else None
-    saved = rk4_ms - (best["ms_estimate"] if best else sur_ms)
+    ok = [r for r in measured if r["rmse"] <= 0.02]
+    best = max(ok, key=lambda r: r["speedup_vs_gpu_rk4"]) if ok else None
+    saved = rk4_ms - (best["ms"] if best else sur_ms)
     result = {
         "metric": METRIC, "value": n_seeds / (sur_ms * 1e-3), "unit": "seeds/s", "n_gpus": 1, "steps": 1,
         "warmup": 1, "ms_per_step": sur_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -586,7 +594,7 @@ def run_surrogate3d(torch, wl, args):
                           "speedup_vs_gpu_rk4": rk4_ms / sur_ms, "seeds_per_s": n_seeds / (sur_ms * 1e-3),
                           "without_probe_plan": spec_runs["without_probe_plan"]},
             "tolerance_sweep_held_out": sweep, "best_within_2pct": best,
-            "measured_all_seeds_first_window": measured,
+            "measured_all_seeds_first_window": measured, "displacement_disagreement_quantiles_50_90_99": disp_q,
             "ftle_rmse_over_range_mean_gate_pass": mean("ftle_gate", "rmse"),
             "ftle_rmse_over_range_mean_jac_1": mean("jac1", "rmse"),
             "ms_surrogate_only_gate_pass": jm4, "ms_surrogate_only_jac_1": mean("jac1", "ms"),
