@@ -10,7 +10,9 @@
 
 Checked: the ensemble's FTLE (mean Jacobian) within 2% RMSE of the field range against RK4;
 a forced fallback (negative tolerance) equal to RK4 bit for bit; the specified gate (FTLE
-disagreement 1e-3 of the range) giving a composite field within the bar.
+disagreement 1e-3 of the range) giving a composite field within the bar; the fastest gated
+setting the bench reports (displacement gate with one Jacobian member: 0.03 in 2D, 1.0 in 3D,
+domain units) within the bar, its fallback seeds equal to RK4.
 """
 
 import math
@@ -49,6 +51,7 @@ def test_surrogate2d_config_four_members():
     cy, cx = torch.meshgrid(ys, xs, indexing="ij")
     seeds = torch.stack([cx.reshape(-1), cy.reshape(-1)], 1).contiguous()
     sf = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], gate="ftle", tol=1e-3)
+    sf_d = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], gate="displacement", tol=0.03, jacobian_members=1)
     for t0 in (0.0, 2.5, 5.0, 7.5):
         ref = sf.rk4_ftle(seeds, t0)
         f, dis = sf.predict(seeds, t0)
@@ -60,6 +63,12 @@ def test_surrogate2d_config_four_members():
               f"rmse/range {err_c:.4f}", flush=True)
         assert err <= 0.02 and err_c <= err + 1e-12
         assert torch.equal(res.ftle[res.fallback], ref[res.fallback])
+        # the fastest gated setting of the bench: displacement gate 0.03, one Jacobian member
+        res_d = sf_d.ftle(seeds, t0)
+        err_d = SF.relative_rmse(res_d.ftle, ref)
+        print(f"[surrogate2d] t0 {t0}: displacement gate 0.03: fallback {res_d.fallback.float().mean().item():.4f}, "
+              f"rmse/range {err_d:.4f}", flush=True)
+        assert err_d <= 0.02 and torch.equal(res_d.ftle[res_d.fallback], ref[res_d.fallback])
     sf.tol = -1.0
     forced = sf.ftle(seeds, 2.5)
     assert bool(forced.fallback.all()) and torch.equal(forced.ftle, sf.rk4_ftle(seeds, 2.5))
@@ -89,6 +98,8 @@ def test_surrogate3d_config_four_members():
     idx = torch.randperm(seeds.shape[0], generator=g, device="cuda")[: seeds.shape[0] // 100].sort().values
     held = seeds[idx].contiguous()
     sf = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], gate="ftle", tol=1e-3, chunk=1 << 19)
+    sf_d = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], gate="displacement", tol=1.0, jacobian_members=1,
+                            chunk=1 << 19)
     for t0 in wl["t0s"]:
         ref = sf.rk4_ftle(held, t0)
         f, dis = sf.predict(seeds, t0)  # the whole seed grid (the gate's range is the window's)
@@ -100,6 +111,13 @@ def test_surrogate3d_config_four_members():
         print(f"[surrogate3d] t0 {t0}: ensemble FTLE rmse/range {err:.4f} on the held-out 1%; gate 1e-3: "
               f"fallback {bad.float().mean().item():.3f}, rmse/range {err_c:.4f}", flush=True)
         assert err <= 0.02 and err_c <= err + 1e-12
+        # the fastest gated setting of the bench: displacement gate 1.0 (domain units), one
+        # Jacobian member, on the held-out seeds (probe plan)
+        res_d = sf_d.ftle(held, t0)
+        err_d = SF.relative_rmse(res_d.ftle, ref)
+        print(f"[surrogate3d] t0 {t0}: displacement gate 1.0: fallback {res_d.fallback.float().mean().item():.4f}, "
+              f"rmse/range {err_d:.4f}", flush=True)
+        assert err_d <= 0.02 and torch.equal(res_d.ftle[res_d.fallback], ref[res_d.fallback])
     sf.tol = -1.0
     forced = sf.ftle(held, wl["t0s"][1])
     assert bool(forced.fallback.all()) and torch.equal(forced.ftle, sf.rk4_ftle(held, wl["t0s"][1]))
