@@ -449,14 +449,18 @@ def build_surrogate3d(torch, n=192, ns=256, t0s=(0.0, 2.5, 5.0, 7.5), members=4,
                         "dtype": "bf16 MFMA, fp32 accumulate"})
 
 
-def run_surrogate3d(torch, wl, args):
-    """3D surrogate FTLE fields against plain GPU RK4: accuracy on the held-out 1% of
-    the seeds, time for all 256^3 seeds of a window (both with HIP events)."""
+def surrogate3d_measure(torch, wl, args, full=True):
+    """3D surrogate FTLE fields against plain GPU RK4 on all 256^3 seeds of the first window
+    (times with HIP events; accuracy against the full RK4 field): the specified gate, the
+    surrogate alone, and the gated settings end to end.  ``full`` adds the node-record RK4,
+    the specified gate without the probe plan, the held-out 1% of all four windows and more
+    tolerances.  Returns the summary object and the RK4 figures."""
     from pyftle_amd import surrogate as S
 
     SF = wl["SF"]
     field = wl["field"]
     L = 2.0 * math.pi
+    t0w = wl["t0s"][0]
     t = time.perf_counter()
     X, D = SF.make_training_data(field, wl["n_traj"], wl["T"], wl["h"], (-1.25, 11.25), (0.0,) * 3, (L,) * 3,
                                  n_bins=400)
@@ -475,9 +479,6 @@ def run_surrogate3d(torch, wl, args):
     cz, cy, cx = torch.meshgrid(sax, sax, sax, indexing="ij")
     seeds = torch.stack([cx.reshape(-1), cy.reshape(-1), cz.reshape(-1)], 1).contiguous()
     del cx, cy, cz
-    g = torch.Generator(device=dev).manual_seed(1)
-    check_idx = torch.randperm(seeds.shape[0], generator=g, device=dev)[: seeds.shape[0] // 100].sort().values
-    check_seeds = seeds[check_idx].contiguous()
     sf = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], chunk=1 << 19)
 
     def timed(fn, reps=1):
@@ -493,74 +494,69 @@ def run_surrogate3d(torch, wl, args):
         return out, a.elapsed_time(b) / reps
 
     n_seeds = seeds.shape[0]
-    _, rk4_node_ms = timed(lambda: sf.rk4_ftle(seeds, wl["t0s"][0]))
+    rk4 = {}
+    if full:
+        _, rk4["ms_node_records"] = timed(lambda: sf.rk4_ftle(seeds, t0w))
     # the fastest plain RK4 of this build: cell-pair records, in a ring holding the snapshots
     # of the timed window only (the whole stored range would take 160 GB as cell pairs)
     dts, t_first = 10.0 / 64.0, -1.25
-    k0 = int(math.floor((wl["t0s"][0] - t_first) / dts))
+    k0 = int(math.floor((t0w - t_first) / dts))
     n_win = int(math.ceil(wl["T"] / dts)) + 2
     cell_field = _abc_unsteady_field_on_device(torch, wl["config"]["grid"][0], t_first + k0 * dts, n_win, dts,
                                                cell=True)
     sf_cell = SF.SurrogateFTLE(ens, cell_field, wl["T"], wl["h"], chunk=1 << 19)
-    ref_cell, rk4_ms = timed(lambda: sf_cell.rk4_ftle(seeds, wl["t0s"][0]))
-    ref_node = sf.rk4_ftle(seeds, wl["t0s"][0])
-    # the two rings start at different times, so the time weights round differently
-    layout_diff = ((ref_cell - ref_node).abs().max() / (ref_node.max() - ref_node.min())).item()
-    del sf_cell, ref_node
+    ref_cell, rk4_ms = timed(lambda: sf_cell.rk4_ftle(seeds, t0w))
+    rk4.update({"ms_all_seeds_one_window": rk4_ms, "seeds_per_s": n_seeds / (rk4_ms * 1e-3),
+                "records": "cell pairs (ring of the timed window's snapshots)"})
+    if full:
+        ref_node = sf.rk4_ftle(seeds, t0w)
+        # the two rings start at different times, so the time weights could round differently
+        rk4["max_ftle_diff_over_range_node_vs_cell"] = \
+            ((ref_cell - ref_node).abs().max() / (ref_node.max() - ref_node.min())).item()
+        del ref_node
+    del sf_cell
     torch.cuda.empty_cache()
-    log(f"plain RK4, all {n_seeds} seeds, one window: {rk4_ms:.1f} ms with cell-pair records "
-        f"({rk4_node_ms:.1f} ms node records; max FTLE difference / range {layout_diff:.1e})")
-    per_t0 = []
+    log(f"plain RK4, all {n_seeds} seeds, one window: {rk4_ms:.1f} ms with cell-pair records")
+    # the surrogate alone on every seed of the window, against the full RK4 field: every member's
+    # Jacobian (the FTLE gate's pass) and one member's (the displacement gate's pass)
     sf_ftle = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], gate="ftle", chunk=1 << 19)
     sf_disp = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], gate="displacement", jacobian_members=1,
                                chunk=1 << 19)
-    tols = (1e-3, 1e-2, 3e-2, 1e-1)
-    for t0 in wl["t0s"]:
-        ref = sf.rk4_ftle(check_seeds, t0)
-        row = {"t0": t0}
-        (f_all, dis), ms = timed(lambda: sf_ftle.predict(seeds, t0))
-        f_j, d_j = f_all[check_idx], dis[check_idx]
-        row["ftle_gate"] = {"ms": ms, "rmse": SF.relative_rmse(f_j, ref)}
-        for tol in tols:  # the composite field on the held-out seeds: RK4 where the gate trips
-            bad = d_j > tol
-            comp = torch.where(bad, ref, f_j)
-            row[f"tol{tol:g}"] = {"fallback": bad.float().mean().item(), "rmse": SF.relative_rmse(comp, ref)}
-        del f_all, dis
-        (f1, _), ms1 = timed(lambda: sf_disp.predict(seeds, t0))
-        row["jac1"] = {"ms": ms1, "rmse": SF.relative_rmse(f1[check_idx], ref)}
-        del f1
-        log(f"t0 {t0}: " + ", ".join(f"{k} {v}" for k, v in row.items() if k != "t0"))
-        per_t0.append(row)
-    mean = lambda key, sub: sum(r[key][sub] for r in per_t0) / len(per_t0)  # noqa: E731
+    (f4, _), jm4 = timed(lambda: sf_ftle.predict(seeds, t0w))
+    rmse4 = SF.relative_rmse(f4, ref_cell)
+    del f4
+    (f1, dsp), jm1 = timed(lambda: sf_disp.predict(seeds, t0w))
+    rmse1 = SF.relative_rmse(f1, ref_cell)
+    sub = dsp[torch.randperm(dsp.numel(), device=dev)[:1 << 20]].float()
+    disp_q = torch.quantile(sub, torch.tensor([0.5, 0.9, 0.99], device=dev)).tolist()
+    del f1, dsp, sub
+    log(f"surrogate alone: every member's Jacobian {jm4:.1f} ms (rmse/range {rmse4:.4f}), one member's "
+        f"{jm1:.1f} ms (rmse/range {rmse1:.4f}); displacement disagreement p50/p90/p99 {disp_q}")
     fpq = S.mlp_flops_per_query(spec)
-    jm4 = mean("ftle_gate", "ms")
-    # the specified setting end to end on every seed of one window: FTLE gate at 1e-3, the
-    # fallback on the cell-pair ring (the timed plain RK4's records); the probe plan (the
-    # default) sends a tile to RK4 whole when more than half of its probed seeds fail
+    rows4 = n_seeds * wl["members"] * 4
+    # the specified setting end to end: FTLE gate at 1e-3, the fallback on the cell-pair ring;
+    # the probe plan (the default) sends a tile to RK4 whole when more than half of its probed
+    # seeds fail
     spec_runs = {}
-    for name, probe in (("planned", 64), ("without_probe_plan", 0)):
+    for name, probe in (("planned", 64), ("without_probe_plan", 0)) if full else (("planned", 64),):
         sf_spec = SF.SurrogateFTLE(ens, cell_field, wl["T"], wl["h"], gate="ftle", tol=1e-3, chunk=1 << 19,
                                    probe=probe)
-        share = sf_spec.tile_fallback_max
-        res, ms = timed(lambda: sf_spec.ftle(seeds, wl["t0s"][0]))
+        res, ms = timed(lambda: sf_spec.ftle(seeds, t0w))
         spec_runs[name] = {"ms": ms, "fallback_fraction": res.fallback.float().mean().item(),
                            "tile_rk4_fraction": torch.isnan(res.spread).float().mean().item(),
                            "speedup_vs_gpu_rk4": rk4_ms / ms}
         log(f"specified setting ({name}): {ms:.1f} ms, fallback {spec_runs[name]['fallback_fraction']:.4f}")
         del res
-    # looser gates end to end on every seed of the same window, judged by the full RK4 field: the
-    # FTLE gate (every member's Jacobian) and the displacement gate (one Jacobian member, domain units)
-    _, dsp = sf_disp.predict(seeds, wl["t0s"][0])
-    sub = dsp[torch.randperm(dsp.numel(), device=dev)[:1 << 20]].float()
-    disp_q = torch.quantile(sub, torch.tensor([0.5, 0.9, 0.99], device=dev)).tolist()
-    del dsp, sub
-    log(f"displacement disagreement p50/p90/p99 {disp_q}")
+    share = sf_spec.tile_fallback_max
+    # looser gates end to end on every seed of the same window, judged by the full RK4 field
+    gates = ((("ftle", 3e-2), ("ftle", 1e-1), ("displacement", 0.1), ("displacement", 0.3), ("displacement", 1.0))
+             if full else (("ftle", 1e-1), ("displacement", 0.3), ("displacement", 1.0)))
     measured = []
-    for gate, tol in (("ftle", 3e-2), ("ftle", 1e-1), ("displacement", 0.1), ("displacement", 0.3), ("displacement", 1.0)):
+    for gate, tol in gates:
         sf_spec = SF.SurrogateFTLE(ens, cell_field, wl["T"], wl["h"], gate=gate, tol=tol, chunk=1 << 19,
                                    jacobian_members=1 if gate == "displacement" else None)
-        res, ms = timed(lambda: sf_spec.ftle(seeds, wl["t0s"][0]))
-        row = {"gate": gate, "tol": tol, "t0": wl["t0s"][0], "ms": ms,
+        res, ms = timed(lambda: sf_spec.ftle(seeds, t0w))
+        row = {"gate": gate, "tol": tol, "t0": t0w, "ms": ms,
                "fallback_fraction": res.fallback.float().mean().item(),
                "tile_rk4_fraction": torch.isnan(res.spread).float().mean().item(),
                "rmse": SF.relative_rmse(res.ftle, ref_cell), "speedup_vs_gpu_rk4": rk4_ms / ms}
@@ -570,45 +566,76 @@ def run_surrogate3d(torch, wl, args):
         del res
     del cell_field, sf_spec, ref_cell
     torch.cuda.empty_cache()
+    summary = {}
+    if full:
+        # the FTLE gate on the held-out 1% of the seeds of all four windows (node-record RK4):
+        # time estimated as the gate's pass plus RK4 on the fallback share
+        g = torch.Generator(device=dev).manual_seed(1)
+        check_idx = torch.randperm(n_seeds, generator=g, device=dev)[: n_seeds // 100].sort().values
+        check_seeds = seeds[check_idx].contiguous()
+        tols = (1e-3, 1e-2, 3e-2, 1e-1)
+        per_t0 = []
+        for t0 in wl["t0s"]:
+            ref = sf.rk4_ftle(check_seeds, t0)
+            f_all, dis = sf_ftle.predict(seeds, t0)
+            f_j, d_j = f_all[check_idx], dis[check_idx]
+            row = {"t0": t0, "ftle_gate_rmse": SF.relative_rmse(f_j, ref)}
+            for tol in tols:
+                bad = d_j > tol
+                row[f"tol{tol:g}"] = {"fallback": bad.float().mean().item(),
+                                      "rmse": SF.relative_rmse(torch.where(bad, ref, f_j), ref)}
+            del f_all, dis
+            f1, _ = sf_disp.predict(seeds, t0)
+            row["jac1_rmse"] = SF.relative_rmse(f1[check_idx], ref)
+            del f1
+            log(f"t0 {t0} (held-out 1%): " + ", ".join(f"{k} {v}" for k, v in row.items() if k != "t0"))
+            per_t0.append(row)
+        sweep = []
+        for tol in tols:
+            f = sum(r[f"tol{tol:g}"]["fallback"] for r in per_t0) / len(per_t0)
+            est = jm4 + f * rk4_ms
+            sweep.append({"tol": tol, "fallback_fraction": f,
+                          "rmse_mean": sum(r[f"tol{tol:g}"]["rmse"] for r in per_t0) / len(per_t0),
+                          "rmse_max": max(r[f"tol{tol:g}"]["rmse"] for r in per_t0), "ms_estimate": est,
+                          "speedup_vs_gpu_rk4_estimate": rk4_ms / est})
+        summary.update({"tolerance_sweep_held_out": sweep, "per_t0_held_out": per_t0})
     sur_ms, fb = spec_runs["planned"]["ms"], spec_runs["planned"]["fallback_fraction"]
-    rows4 = n_seeds * wl["members"] * 4
-    # estimated time at each tolerance: the gate's pass plus RK4 on the fallback fraction
-    sweep = []
-    for tol in tols:
-        f = sum(r[f"tol{tol:g}"]["fallback"] for r in per_t0) / len(per_t0)
-        est = jm4 + f * rk4_ms
-        sweep.append({"tol": tol, "fallback_fraction": f, "rmse_mean": mean(f"tol{tol:g}", "rmse"),
-                      "rmse_max": max(r[f"tol{tol:g}"]["rmse"] for r in per_t0), "ms_estimate": est,
-                      "speedup_vs_gpu_rk4_estimate": rk4_ms / est})
     ok = [r for r in measured if r["rmse"] <= 0.02]
     best = max(ok, key=lambda r: r["speedup_vs_gpu_rk4"]) if ok else None
     saved = rk4_ms - (best["ms"] if best else sur_ms)
+    specified = {"tol": 1e-3, "gate": "ftle", "ms": sur_ms, "fallback_fraction": fb,
+                 "tile_rk4_fraction": spec_runs["planned"]["tile_rk4_fraction"], "tile_fallback_max": share,
+                 "speedup_vs_gpu_rk4": rk4_ms / sur_ms, "seeds_per_s": n_seeds / (sur_ms * 1e-3)}
+    if full:
+        specified["without_probe_plan"] = spec_runs["without_probe_plan"]
+    summary.update({
+        "workload": "surrogate3d", "seeds": n_seeds, "window_t0": t0w, "members": wl["members"],
+        "network": "64 Fourier freqs on (x,y,z,t0), 6x512 SiLU -> 3",
+        "gate": "ftle: max over members |FTLE_member - FTLE_ensemble| / (FTLE range of the window's probe); "
+                "displacement: max member deviation of the displacement, domain units (the domain is 2 pi)",
+        "specified": specified, "best_within_2pct": best, "measured_all_seeds_first_window": measured,
+        "displacement_disagreement_quantiles_50_90_99": disp_q,
+        "surrogate_alone": {"ftle": {"ms": jm4, "rmse": rmse4, "jacobian_members": wl["members"],
+                                     "speedup_vs_gpu_rk4": rk4_ms / jm4},
+                            "displacement": {"ms": jm1, "rmse": rmse1, "jacobian_members": 1,
+                                             "speedup_vs_gpu_rk4": rk4_ms / jm1}},
+        "mfma": {"achieved": rows4 * fpq / (jm4 * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
+                 "frac": rows4 * fpq / (jm4 * 1e-3) / 1e12 / 2500.0, "flops_per_query": fpq,
+                 "kernel": "fused_mlp_kernel<512,4,3,3> (every member with its 3 tangents)"},
+        "rk4": rk4, "train_s": t_train, "data_s": t_data,
+        "break_even_windows": (t_train + t_data) / (saved * 1e-3) if saved > 0 else None})
+    return summary, seeds
+
+
+def run_surrogate3d(torch, wl, args):
+    """The 3D surrogate as its own bench line: value = seeds/s of the specified setting."""
+    summ, seeds = surrogate3d_measure(torch, wl, args, full=True)
+    sp = summ["specified"]
     result = {
-        "metric": METRIC, "value": n_seeds / (sur_ms * 1e-3), "unit": "seeds/s", "n_gpus": 1, "steps": 1,
-        "warmup": 1, "ms_per_step": sur_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "metric": METRIC, "value": sp["seeds_per_s"], "unit": "seeds/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": sp["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16", "data": "synthetic", "config": dict(wl["config"], fallback_tol=1e-3, gate="ftle"),
-        "surrogate": {
-            "gate": "ftle: max over members |FTLE_member - FTLE_ensemble| / (FTLE range of the window's probe)",
-            "specified": {"tol": 1e-3, "ms": sur_ms, "fallback_fraction": fb,
-                          "tile_rk4_fraction": spec_runs["planned"]["tile_rk4_fraction"], "tile_fallback_max": share,
-                          "speedup_vs_gpu_rk4": rk4_ms / sur_ms, "seeds_per_s": n_seeds / (sur_ms * 1e-3),
-                          "without_probe_plan": spec_runs["without_probe_plan"]},
-            "tolerance_sweep_held_out": sweep, "best_within_2pct": best,
-            "measured_all_seeds_first_window": measured, "displacement_disagreement_quantiles_50_90_99": disp_q,
-            "ftle_rmse_over_range_mean_gate_pass": mean("ftle_gate", "rmse"),
-            "ftle_rmse_over_range_mean_jac_1": mean("jac1", "rmse"),
-            "ms_surrogate_only_gate_pass": jm4, "ms_surrogate_only_jac_1": mean("jac1", "ms"),
-            "per_t0": per_t0,
-            "mfma": {"achieved": rows4 * fpq / (jm4 * 1e-3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
-                     "frac": rows4 * fpq / (jm4 * 1e-3) / 1e12 / 2500.0, "flops_per_query": fpq,
-                     "kernel": "fused_mlp_kernel<512,4,3,3> (every member with its 3 tangents)"},
-            "train_s": t_train, "data_s": t_data,
-            "break_even_windows": (t_train + t_data) / (saved * 1e-3) if saved > 0 else None},
-        "rk4": {"ms_all_seeds_one_window": rk4_ms, "seeds_per_s": n_seeds / (rk4_ms * 1e-3),
-                "records": "cell pairs (ring of the timed window's snapshots)",
-                "ms_node_records": rk4_node_ms, "max_ftle_diff_over_range_node_vs_cell": layout_diff},
-        "speedup_vs_gpu_rk4": rk4_ms / sur_ms,
-        "surrogate_only_speedup_vs_gpu_rk4": {"gate_pass": rk4_ms / jm4, "jac_1": rk4_ms / mean("jac1", "ms")},
+        "surrogate": summ,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(_abc_cpu_workload(torch, wl, seeds), budget_s=15.0),
     }
     print(json.dumps(result), flush=True)
@@ -1260,6 +1287,8 @@ def main():
                     help="only measure the issue cost (cycles) of MFMA, FP32 multiply, exp2 and reciprocal, "
                          "alone and with an MFMA wave and a VALU wave sharing a SIMD")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-surrogate3d", action="store_true",
+                    help="default line: skip its 3D surrogate leg (the 2D leg stays)")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="abc128: skip the surrogate leg (training and timing the 2D surrogate, ~1 min)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy check")
@@ -1452,6 +1481,13 @@ def main():
         summ, plan = surrogate2d_measure(torch, build_surrogate2d(torch), args)
         plan.close()
         result["surrogate"] = summ
+        del summ, plan
+        torch.cuda.empty_cache()
+        if not args.no_surrogate3d:
+            # and the 3D surrogate (unsteady ABC 192^3, 4 members of 6x512) on one 256^3-seed window
+            log("surrogate leg (3D unsteady ABC, 4 members of 6x512, 16.8M seeds)")
+            summ3, _ = surrogate3d_measure(torch, build_surrogate3d(torch), args, full=False)
+            result["surrogate3d"] = summ3
     print(json.dumps(result), flush=True)
 
 
