@@ -1147,7 +1147,14 @@ def load_traffic(name):
 # ---------------------------------------------------------------------------
 
 def run_batch(torch, wl, args):
-    """64 uneven double-gyre windows in one persistent, graph-captured launch."""
+    """64 uneven double-gyre windows in one persistent, graph-captured launch (its bench line)."""
+    print(json.dumps(measure_batch(torch, wl, args)), flush=True)
+
+
+def measure_batch(torch, wl, args, with_cpu=True):
+    """64 uneven double-gyre windows in one persistent, graph-captured launch: timed replays,
+    the roofline of the binding unit, parity with single-window launches and FP64, and
+    (with_cpu) the CPU baseline; returns the bench-line object."""
     from pyftle_amd import window as W
     from pyftle_amd.batch import BatchConfig, BatchPlan
     from pyftle_amd.tuning import best_batch_config
@@ -1247,7 +1254,7 @@ def run_batch(torch, wl, args):
         result["accuracy"] = {"batch_equals_single_window_launches": same,
                               "fp64_fields_checked": checked, "max_abs_err_over_range": worst}
     cpu = None
-    if not args.no_cpu:
+    if with_cpu and not args.no_cpu:
         # the reference CPU path on the batch's most common window shape (512x256 gyre,
         # T = 10), converted to the batch's seeds/s through its mean steps per seed
         cpu = cpu_baseline(_gyre_cpu_workload(torch, {"config": {"grid": [512, 256]}, "h": 0.05, "T": 10.0}),
@@ -1257,110 +1264,17 @@ def run_batch(torch, wl, args):
             cpu["sample"] += (f"; batch seeds/s = seed-steps/s x {wl['n_seeds']} seeds / "
                               f"{wl['seed_steps']} seed-steps of the batch")
     result["cpu_baseline"] = cpu
-    print(json.dumps(result), flush=True)
     plan.close()
+    return result
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="abc128", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-graph", action="store_true",
-                    help="batch64: plain launches instead of the captured hipGraph (for per-dispatch counters)")
-    ap.add_argument("--surrogate-epochs", type=int, default=0,
-                    help="surrogate2d / surrogate3d: training epochs (0 = the workload's default)")
-    ap.add_argument("--surrogate-depth", type=int, default=0,
-                    help="surrogate2d: hidden layers of width 256 (0 = the 5 of the specified network; "
-                         "other depths are an accuracy/speed exploration)")
-    ap.add_argument("--train-batch", type=int, default=0, help="mlp_train: rows per step (0 = 16384)")
-    ap.add_argument("--train-graph", type=int, default=0, help="mlp_train: replay each step as a HIP graph")
-    ap.add_argument("--train-shape", default="2d", choices=["2d", "3d"], help="mlp_train: the 2D or the 3D network")
-    ap.add_argument("--train-layered", action="store_true", help="mlp_train: the layer-by-layer kernels")
-    ap.add_argument("--train-splits", type=int, default=0,
-                    help="mlp_train: split-K of the fused step's weight-gradient GEMMs (0 = its default)")
-    ap.add_argument("--tile-occupancy", action="store_true",
-                    help="abc128 / gyre512: only measure how much of the window an LDS tile per work-group "
-                         "could serve (share of particles inside their work-group's tile, by step)")
-    ap.add_argument("--issue-probe", action="store_true",
-                    help="only measure the issue cost (cycles) of MFMA, FP32 multiply, exp2 and reciprocal, "
-                         "alone and with an MFMA wave and a VALU wave sharing a SIMD")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-surrogate3d", action="store_true",
-                    help="default line: skip its 3D surrogate leg (the 2D leg stays)")
-    ap.add_argument("--no-surrogate", action="store_true",
-                    help="abc128: skip the surrogate leg (training and timing the 2D surrogate, ~1 min)")
-    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy check")
-    ap.add_argument("--seed-order", default="auto", choices=["auto", "morton", "none"],
-                    help="abc128 / gyre512: process seed groups in Morton order of their cells (results are "
-                         "identical; computed inside every timed step); auto = from 2^18 seed groups")
-    ap.add_argument("--records", default="auto", choices=["auto", "node", "cell"],
-                    help="record layout of the FP32 fields of abc128, gyre512, batch64 and the 2D surrogate "
-                         "workloads: one record per grid node, one per cell holding its corners, or auto "
-                         "(cell records)")
-    ap.add_argument("--groups-per-block", type=int, default=0)
-    ap.add_argument("--lane-order", type=int, default=-1)
-    ap.add_argument("--xcd-remap", type=int, default=-1)
-    ap.add_argument("--copy-calibration", type=int, default=0,
-                    help="only copy this many bytes device-to-device 3 times (counter calibration)")
-    args = ap.parse_args()
-
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: every FTLE seed group is independent and this build "
-              "runs on a single MI355X; --gpus above 1 is not supported.", file=sys.stderr)
-        sys.exit(2)
-
-    import torch
-
-    if args.copy_calibration:
-        n = args.copy_calibration // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes": n * 4}), flush=True)
-        return
-
-    from pyftle_amd import _native
+def run_window(torch, wl, args, with_cpu=True):
+    """One single-window workload (abc128, gyre512): timed launches, the roofline of the binding
+    unit, accuracy against FP64 and the oracle, and (with_cpu) the CPU baseline; returns the
+    bench-line object."""
     from pyftle_amd import window as W
     from pyftle_amd.tuning import best_config, shape_class
 
-    _native.load()
-    torch.cuda.set_device(0)
-    if args.issue_probe:
-        print(json.dumps({"issue_probe": W.probe_issue_rates()}), flush=True)
-        return
-    log(f"building workload {args.workload}")
-    kw = {}
-    if args.workload in ("surrogate2d", "surrogate3d") and args.surrogate_epochs > 0:
-        kw["epochs"] = args.surrogate_epochs
-    if args.workload == "surrogate2d" and args.surrogate_depth > 0:
-        kw["depth"] = args.surrogate_depth
-    if args.workload == "mlp_train":
-        kw["shape"] = args.train_shape
-        if args.train_batch > 0:
-            kw["batch"] = args.train_batch
-    global RECORDS
-    RECORDS = args.records
-    wl = WORKLOADS[args.workload](torch, **kw)
-    if "config" in wl and args.workload not in ("surrogate3d", "surrogate3d_infer", "mlp_train"):
-        wl["config"]["records"] = _records_name(wl.get("dim", 2))
-    if wl["name"] == "batch64":
-        return run_batch(torch, wl, args)
-    if wl["name"] == "mlp_train":
-        return run_mlp_train(torch, wl, args)
-    if args.tile_occupancy:
-        return run_tile_occupancy(torch, wl, args)
-    if wl["name"] == "surrogate2d":
-        return run_surrogate(torch, wl, args)
-    if wl["name"] == "surrogate_infer":
-        return run_surrogate_infer(torch, wl, args)
-    if wl["name"] == "surrogate3d_infer":
-        return run_surrogate3d_infer(torch, wl, args)
-    if wl["name"] == "surrogate3d":
-        return run_surrogate3d(torch, wl, args)
     cfg = best_config(shape_class(wl["dim"], wl["n_seeds"], wl["time_mode"], records=_records_name(wl["dim"])))
     if args.groups_per_block > 0:
         cfg.groups_per_block = args.groups_per_block
@@ -1470,9 +1384,115 @@ def main():
         acc.update(accuracy_vs_oracle(torch, wl, ftle))
         result["accuracy"] = acc
     result["cpu_baseline"] = None
-    if not args.no_cpu:
+    if with_cpu and not args.no_cpu:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(wl)
+    return result
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="abc128", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batch64: plain launches instead of the captured hipGraph (for per-dispatch counters)")
+    ap.add_argument("--surrogate-epochs", type=int, default=0,
+                    help="surrogate2d / surrogate3d: training epochs (0 = the workload's default)")
+    ap.add_argument("--surrogate-depth", type=int, default=0,
+                    help="surrogate2d: hidden layers of width 256 (0 = the 5 of the specified network; "
+                         "other depths are an accuracy/speed exploration)")
+    ap.add_argument("--train-batch", type=int, default=0, help="mlp_train: rows per step (0 = 16384)")
+    ap.add_argument("--train-graph", type=int, default=0, help="mlp_train: replay each step as a HIP graph")
+    ap.add_argument("--train-shape", default="2d", choices=["2d", "3d"], help="mlp_train: the 2D or the 3D network")
+    ap.add_argument("--train-layered", action="store_true", help="mlp_train: the layer-by-layer kernels")
+    ap.add_argument("--train-splits", type=int, default=0,
+                    help="mlp_train: split-K of the fused step's weight-gradient GEMMs (0 = its default)")
+    ap.add_argument("--tile-occupancy", action="store_true",
+                    help="abc128 / gyre512: only measure how much of the window an LDS tile per work-group "
+                         "could serve (share of particles inside their work-group's tile, by step)")
+    ap.add_argument("--issue-probe", action="store_true",
+                    help="only measure the issue cost (cycles) of MFMA, FP32 multiply, exp2 and reciprocal, "
+                         "alone and with an MFMA wave and a VALU wave sharing a SIMD")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-other-workloads", action="store_true",
+                    help="default line: skip its gyre512 and batch64 objects")
+    ap.add_argument("--no-surrogate3d", action="store_true",
+                    help="default line: skip its 3D surrogate leg (the 2D leg stays)")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="abc128: skip the surrogate leg (training and timing the 2D surrogate, ~1 min)")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 accuracy check")
+    ap.add_argument("--seed-order", default="auto", choices=["auto", "morton", "none"],
+                    help="abc128 / gyre512: process seed groups in Morton order of their cells (results are "
+                         "identical; computed inside every timed step); auto = from 2^18 seed groups")
+    ap.add_argument("--records", default="auto", choices=["auto", "node", "cell"],
+                    help="record layout of the FP32 fields of abc128, gyre512, batch64 and the 2D surrogate "
+                         "workloads: one record per grid node, one per cell holding its corners, or auto "
+                         "(cell records)")
+    ap.add_argument("--groups-per-block", type=int, default=0)
+    ap.add_argument("--lane-order", type=int, default=-1)
+    ap.add_argument("--xcd-remap", type=int, default=-1)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only copy this many bytes device-to-device 3 times (counter calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every FTLE seed group is independent and this build "
+              "runs on a single MI355X; --gpus above 1 is not supported.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if args.copy_calibration:
+        n = args.copy_calibration // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4}), flush=True)
+        return
+
+    from pyftle_amd import _native
+    from pyftle_amd import window as W
+    from pyftle_amd.tuning import best_config, shape_class
+
+    _native.load()
+    torch.cuda.set_device(0)
+    if args.issue_probe:
+        print(json.dumps({"issue_probe": W.probe_issue_rates()}), flush=True)
+        return
+    log(f"building workload {args.workload}")
+    kw = {}
+    if args.workload in ("surrogate2d", "surrogate3d") and args.surrogate_epochs > 0:
+        kw["epochs"] = args.surrogate_epochs
+    if args.workload == "surrogate2d" and args.surrogate_depth > 0:
+        kw["depth"] = args.surrogate_depth
+    if args.workload == "mlp_train":
+        kw["shape"] = args.train_shape
+        if args.train_batch > 0:
+            kw["batch"] = args.train_batch
+    global RECORDS
+    RECORDS = args.records
+    wl = WORKLOADS[args.workload](torch, **kw)
+    if "config" in wl and args.workload not in ("surrogate3d", "surrogate3d_infer", "mlp_train"):
+        wl["config"]["records"] = _records_name(wl.get("dim", 2))
+    if wl["name"] == "batch64":
+        return run_batch(torch, wl, args)
+    if wl["name"] == "mlp_train":
+        return run_mlp_train(torch, wl, args)
+    if args.tile_occupancy:
+        return run_tile_occupancy(torch, wl, args)
+    if wl["name"] == "surrogate2d":
+        return run_surrogate(torch, wl, args)
+    if wl["name"] == "surrogate_infer":
+        return run_surrogate_infer(torch, wl, args)
+    if wl["name"] == "surrogate3d_infer":
+        return run_surrogate3d_infer(torch, wl, args)
+    if wl["name"] == "surrogate3d":
+        return run_surrogate3d(torch, wl, args)
+    result = run_window(torch, wl, args)
     if wl["name"] == "abc128" and not args.no_surrogate:
         # the metric's surrogate half: the 2D flow-map surrogate against plain GPU RK4
         log("surrogate leg (2D double gyre, 4 members of 5x256, 67.1M seed-windows)")
@@ -1488,6 +1508,20 @@ def main():
             log("surrogate leg (3D unsteady ABC, 4 members of 6x512, 16.8M seeds)")
             summ3, _ = surrogate3d_measure(torch, build_surrogate3d(torch), args, full=False)
             result["surrogate3d"] = summ3
+            del summ3
+            torch.cuda.empty_cache()
+    if args.workload == "abc128" and not args.no_other_workloads:
+        # the other plain-RK4 configurations, so that every figure of the headline is on this line:
+        # the 2D double-gyre window (C0) and the 64-field batch (C2), without their CPU baselines
+        keep = ("config", "ms_per_step", "value", "unit", "seed_steps_per_s", "roofline", "launch", "accuracy")
+        for name, build, measure in (("gyre512", build_gyre, run_window), ("batch64", build_batch64, measure_batch)):
+            log(f"workload {name}")
+            w2 = build(torch)
+            w2["config"]["records"] = _records_name(2)
+            r = measure(torch, w2, args, with_cpu=False)
+            result[name] = {k: r[k] for k in keep if k in r}
+            del w2, r
+            torch.cuda.empty_cache()
     print(json.dumps(result), flush=True)
 
 
