@@ -16,11 +16,16 @@
 // 2. Weight gradients dW_l = DZT_l HT_{l-1}^T: the BF16 MFMA GEMM of mlp.hip with both
 //    operands K-contiguous, split over K (FP32 partials).
 // 3. train_reduce_kernel: every parameter's gradient summed over splits or tiles in a fixed
-//    order (deterministic), the loss accumulated, the batch offset advanced.
-// Adam (pyftle_mlp_adam_scheduled) and pyftle_mlp_pack_train (forward and transposed
-// fragment-packed bf16 weights for the next step) complete a step.
+//    order (deterministic), the loss accumulated, the batch offset advanced; in
+//    pyftle_mlp_train_step it also applies Adam to each summed gradient and rewrites the
+//    forward and transposed fragment-packed bf16 weights the next step reads
+//    (pyftle_mlp_train_grads leaves that to train_adam_kernel, pyftle_mlp_train_adam).
 //
-// Transposed copies leave LDS as 16-byte stores: a lane takes a feature pair and 8 rows,
+// Transposed copies of the hidden activations and of every dZ leave the registers they are
+// computed in: the four lanes of a quad hold a 4 x 4 block (rows x features) of a tile, which
+// two DPP exchanges inside the quad turn into 4 rows of one feature per lane; one more
+// exchange between neighbouring quads pairs two row tiles into 16-byte stores of 8 rows.
+// The features' copy leaves LDS as 16-byte stores: a lane takes a feature pair and 8 rows,
 // reading the 8 rows in an order rotated by its row octet so that the 64 lanes of a read
 // hit distinct banks, and rotates the words back in registers.
 
@@ -102,6 +107,70 @@ __device__ __forceinline__ void transpose_out(const __bf16 *act, int nfeat, __bf
   }
 }
 
+// DPP move of x (lanes outside the bank mask keep `old`).  Every lane of the wave must be active.
+template <int CTRL, int BANKS = 0xF>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t old, uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)x, CTRL, 0xF, BANKS, false);
+}
+constexpr int kDppXor1 = 0xB1;   // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;   // quad_perm [2,3,0,1]
+constexpr int kDppShl4 = 0x104;  // row_shl:4, lane l reads l + 4 of its row of 16
+constexpr int kDppShr4 = 0x114;  // row_shr:4, lane l reads l - 4
+constexpr int kDppRor8 = 0x128;  // row_ror:8, lane l reads (l + 8) mod 16 of its row: l ^ 8
+
+// The word of lane l ^ 4: banks 0 and 2 of each row of 16 lanes read 4 lanes up, banks 1 and
+// 3 four lanes down.
+__device__ __forceinline__ uint32_t swap_xor4(uint32_t x) {
+  return dpp_u32<kDppShr4, 0xA>(dpp_u32<kDppShl4, 0x5>(0u, x), x);
+}
+
+// v summed over the 16 lanes of its row (lane / 16), in the order of an xor butterfly over
+// lane bits 0, 1, 2, 3 (every lane gets the same bits); DPP moves instead of LDS permutes.
+__device__ __forceinline__ float row16_sum(float v) {
+  v += __builtin_bit_cast(float, dpp_u32<kDppXor1>(0u, __builtin_bit_cast(uint32_t, v)));
+  v += __builtin_bit_cast(float, dpp_u32<kDppXor2>(0u, __builtin_bit_cast(uint32_t, v)));
+  v += __builtin_bit_cast(float, swap_xor4(__builtin_bit_cast(uint32_t, v)));
+  v += __builtin_bit_cast(float, dpp_u32<kDppRor8>(0u, __builtin_bit_cast(uint32_t, v)));
+  return v;
+}
+
+// The 4 x 4 bf16 block of a quad of lanes transposed inside the quad.  On entry lane q
+// (q = lane % 4) holds row q of the block as two words, features (0, 1) in `lo` and (2, 3) in
+// `hi` (the first of each pair in the low half); on return it holds feature q, rows (0, 1) in
+// .x and rows (2, 3) in .y.  First the lanes q and q ^ 2 swap word pairs (DPP quad_perm
+// [2,3,0,1]): lane q then holds features (q & 2, +1) of rows q & 1 (u) and (q & 1) + 2 (v).
+// Then the lanes q and q ^ 1 swap halves (quad_perm [1,0,3,2]); v_perm_b32 picks the low
+// halves (even lanes: own, partner's) or the high halves (odd lanes: partner's, own) of the
+// combined bytes {partner word: 4..7, own word: 0..3}.  Every lane of the wave must be active.
+__device__ __forceinline__ uint2 quad_transpose(uint32_t lo, uint32_t hi, int q) {
+  const bool upper = (q & 2) != 0;
+  const uint32_t x = upper ? lo : hi;
+  const uint32_t y = dpp_u32<kDppXor2>(0u, x);
+  const uint32_t u = upper ? y : lo;
+  const uint32_t v = upper ? hi : y;
+  const uint32_t up = dpp_u32<kDppXor1>(0u, u), vp = dpp_u32<kDppXor1>(0u, v);
+  const uint32_t sel = (q & 1) ? 0x03020706u : 0x05040100u;
+  return uint2{__builtin_amdgcn_perm(up, u, sel), __builtin_amdgcn_perm(vp, v, sel)};
+}
+
+// Row tiles t and t + 1 of lane (r, g) -- rows 16 t + r and 16 (t + 1) + r, features
+// f0 .. f0 + 3 (f0 = 16 ft + 4 g), as bf16x4 h0, h1 -- stored transposed, dst[f * ld + col0 +
+// row], as one 16-byte store per lane: after the quad transposes lane (a = r / 4, q = r % 4)
+// holds rows 4 a .. 4 a + 3 of feature f0 + q in both tiles, and lanes a and a ^ 1 swap one
+// tile's words, so that an even a stores rows 4 a .. 4 a + 7 of tile t and an odd a rows
+// 4 (a - 1) .. 4 (a - 1) + 7 of tile t + 1.  A wave's store covers 32 rows of 16 features.
+__device__ __forceinline__ void store_transposed(bf16x4_t h0, bf16x4_t h1, __bf16 *dst, int64_t ld, int64_t col0,
+                                                 int f0, int t, int r) {
+  const uint2 w0 = __builtin_bit_cast(uint2, h0), w1 = __builtin_bit_cast(uint2, h1);
+  const uint2 o0 = quad_transpose(w0.x, w0.y, r & 3);
+  const uint2 o1 = quad_transpose(w1.x, w1.y, r & 3);
+  const bool odd = (r & 4) != 0;
+  const uint32_t y0 = swap_xor4(odd ? o0.x : o1.x), y1 = swap_xor4(odd ? o0.y : o1.y);
+  const uint4 out = odd ? uint4{y0, y1, o1.x, o1.y} : uint4{o0.x, o0.y, y0, y1};
+  const int row = odd ? 16 * (t + 1) + 4 * ((r >> 2) - 1) : 16 * t + 4 * (r >> 2);
+  *reinterpret_cast<uint4 *>(dst + (int64_t)(f0 + (r & 3)) * ld + col0 + row) = out;
+}
+
 // Per-feature sum over this lane's rows (its RT row tiles), then over the 16 lanes that
 // share the lane's features (lane % 16 = row within a tile): lanes with lane % 16 == 0 write.
 template <int W, int RT>
@@ -117,33 +186,32 @@ __device__ __forceinline__ void feature_sums(const f32x4_t (&acc)[Shape<W, RT>::
       float v = 0.0f;
 #pragma unroll
       for (int t = 0; t < RT; ++t) v += acc[i][t][e];
-      v += __shfl_xor(v, 1);
-      v += __shfl_xor(v, 2);
-      v += __shfl_xor(v, 4);
-      v += __shfl_xor(v, 8);
-      s[e] = v;
+      s[e] = row16_sum(v);
     }
     if (r == 0) *reinterpret_cast<f32x4_t *>(dst + (ft0 + i) * 16 + 4 * g) = f32x4_t{s[0], s[1], s[2], s[3]};
   }
 }
 
-// acc (FP32) as bf16 activation rows of LDS (the layout the MFMA loop reads)
+// acc (FP32) as bf16 activation rows of LDS (the layout the MFMA loop reads) and, transposed,
+// as dst[f * ld + col0 + row]
 template <int W, int RT>
 __device__ __forceinline__ void store_rows(const f32x4_t (&acc)[Shape<W, RT>::kTilesPerWave][RT], __bf16 *act,
-                                           int ft0, int lane_in) {
+                                           __bf16 *dst, int64_t ld, int64_t col0, int ft0, int lane_in) {
   const int lane = opaque_lane(lane_in);
   const int r = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int i = 0; i < Shape<W, RT>::kTilesPerWave; ++i) {
     const int f0 = (ft0 + i) * 16 + 4 * g;
     const int off = act_off<W>(r, f0 >> 3) + (f0 & 7);
+    bf16x4_t h[RT];
 #pragma unroll
     for (int t = 0; t < RT; ++t) {
-      bf16x4_t h;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) h[e] = (__bf16)acc[i][t][e];
-      *reinterpret_cast<bf16x4_t *>(act + off + t * 16 * W) = h;
+      for (int e = 0; e < 4; ++e) h[t][e] = (__bf16)acc[i][t][e];
+      *reinterpret_cast<bf16x4_t *>(act + off + t * 16 * W) = h[t];
     }
+#pragma unroll
+    for (int t = 0; t < RT; t += 2) store_transposed(h[t], h[t + 1], dst, ld, col0, f0, t, r);
   }
 }
 
@@ -153,6 +221,7 @@ template <int W, int RT, int O>
 __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(TrainArgs a) {
   using Sh = Shape<W, RT>;
   constexpr int TPW = Sh::kTilesPerWave, kRows = Sh::kRows;
+  static_assert(RT % 2 == 0, "row tiles leave transposed in pairs");
   __shared__ __attribute__((aligned(16))) __bf16 act[kRows * W];
   __shared__ __attribute__((aligned(16))) float freq_s[kNFreq * kMaxIn];
   __shared__ float red[kWaves][kRows][4];
@@ -216,35 +285,38 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
   // accumulator array is carried around a loop holding two instantiations of the MFMA loop.
   layer_mma_ring<W, RT, kFeat, PD>(wr, layer_bytes<W>(0), layer_bytes<W>(1), act, ring, acc, ft0, lane);
   for (int L = 0;; ++L) {
-    __syncthreads();  // every wave has read this layer's inputs (and their transposed copy is out)
+    __syncthreads();  // every wave has read this layer's inputs
     {
       const float *bl = a.params + b_off<W>(L);
       const int ln = opaque_lane(lane);
       const int r = ln & 15, g = ln >> 4;
       __bf16 *dact = a.DACT + ((((int64_t)L * ntiles + tile) * kWaves + wave) * TPW * RT * 64) * 4;
+      __bf16 *ht = a.HT + (int64_t)L * W * M;
 #pragma unroll
       for (int i = 0; i < TPW; ++i) {
         const int f0 = (ft0 + i) * 16 + 4 * g;
         const f32x4_t bv = *reinterpret_cast<const f32x4_t *>(bl + f0);
         const int off = act_off<W>(r, f0 >> 3) + (f0 & 7);
+        bf16x4_t h[RT];
 #pragma unroll
         for (int t = 0; t < RT; ++t) {
-          bf16x4_t h, d;
+          bf16x4_t d;
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const float z = acc[i][t][e] + bv[e];
             const float s = sigmoid_f(z);
-            h[e] = (__bf16)(z * s);
+            h[t][e] = (__bf16)(z * s);
             d[e] = (__bf16)(s * (1.0f + z * (1.0f - s)));
           }
-          *reinterpret_cast<bf16x4_t *>(act + off + t * 16 * W) = h;
+          *reinterpret_cast<bf16x4_t *>(act + off + t * 16 * W) = h[t];
           *reinterpret_cast<bf16x4_t *>(dact + ((i * RT + t) * 64 + ln) * 4) = d;
         }
+#pragma unroll
+        for (int t = 0; t < RT; t += 2) store_transposed(h[t], h[t + 1], ht, M, col0, f0, t, r);
         __builtin_amdgcn_sched_barrier(0);  // one feature tile at a time: no loads hoisted
       }
     }
     __syncthreads();
-    transpose_out<W, RT>(act, W, a.HT + (int64_t)L * W * M, M, col0, tid);
     if (L + 2 == a.depth) break;
     layer_mma_ring<W, RT, W, PD>(wr, layer_bytes<W>(L + 1), layer_bytes<W>(L + 2), act, ring, acc, ft0, lane);
   }
@@ -356,12 +428,7 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
       for (int o = 0; o < O; ++o)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          float v = ws[o][e];
-          v += __shfl_xor(v, 1);
-          v += __shfl_xor(v, 2);
-          v += __shfl_xor(v, 4);
-          v += __shfl_xor(v, 8);
-          ws[o][e] = v;
+          ws[o][e] = row16_sum(ws[o][e]);
         }
       if (r == 0) {
 #pragma unroll
@@ -374,10 +441,10 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
   // backward through the hidden layers: acc holds dZ of layer l
   for (int l = a.depth - 1;; --l) {
     feature_sums<W, RT>(acc, a.bias_part + ((int64_t)tile * a.depth + l) * W, ft0, lane);
-    store_rows<W, RT>(acc, act, ft0, lane);  // every wave finished reading act before the last barrier
-    __syncthreads();
-    transpose_out<W, RT>(act, W, a.DZT + (int64_t)l * W * M, M, col0, tid);
+    // every wave finished reading act before the last barrier
+    store_rows<W, RT>(acc, act, a.DZT + (int64_t)l * W * M, M, col0, ft0, lane);
     if (l == 0) break;
+    __syncthreads();
     const int lb = tlayer_bytes<W>(l, a.depth);
     const int nlb = l > 1 ? tlayer_bytes<W>(l - 1, a.depth) : layer_bytes<W>(0);
     layer_mma_ring<W, RT, W, PD>(wr, lb, nlb, act, ring, acc, ft0, lane);  // dH of layer l - 1
@@ -465,17 +532,41 @@ struct ReduceArgs {
   AdamParams adam;         // ADAM: the update applied to each summed gradient
 };
 
+// src[k * stride] summed over k = k0, k0 + step, ... below n, added in that order.  The loads
+// go out 16 at a time: the partials were written by the previous launch and come from L2 or
+// memory, so a term-by-term chain would wait one round trip per term.
+__device__ __forceinline__ float strided_sum(const float *src, int64_t stride, int k0, int step, int n) {
+  float s = 0.0f;
+  for (int k = k0; k < n; k += 16 * step) {
+    float v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int kk = k + j * step;
+      v[j] = kk < n ? src[(int64_t)kk * stride] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) s += v[j];
+  }
+  return s;
+}
+
 // Gradients summed in a fixed order: the hidden weights over the GEMM splits (a thread per
 // weight), the rest over the row tiles (16 parameters per work-group, 16 interleaved partial
 // sums per parameter combined in order), the loss by one more work-group.  With ADAM each
 // summed gradient is also applied to its parameter at once (the step's Adam update, as
-// train_adam_kernel would apply it) and the hidden weights' bf16 packings are rewritten.
+// train_adam_kernel would apply it, its schedule evaluated once per work-group) and the hidden
+// weights' bf16 packings are rewritten.
 template <bool ADAM>
 __global__ void __launch_bounds__(256) train_reduce_kernel(ReduceArgs a) {
   const int W = a.W;
+  __shared__ AdamSchedule sched;
+  if constexpr (ADAM) {
+    if (threadIdx.x == 0) sched = adam_schedule(a.adam);
+    __syncthreads();
+  }
   if ((int)blockIdx.x == (int)gridDim.x - 1) {
     // the last work-group: the loss over the row tiles (thread j sums tiles j, j + 256, ...
-    // in order, then thread 0 the 256 partial sums in order), the batch offset and the step
+    // in order, then thread 0 the 256 partial sums in order) and the batch offset
     __shared__ float lp[256];
     float l = 0.0f;
     for (int k = threadIdx.x; k < a.tiles; k += blockDim.x) l += a.loss_part[k];
@@ -498,20 +589,10 @@ __global__ void __launch_bounds__(256) train_reduce_kernel(ReduceArgs a) {
       const int64_t nw = (int64_t)W * fan;
       if (q < woff + nw) {
         const int64_t loc = q - woff;
-        const float *src = a.part + poff + loc;
-        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-        int s = 0;
-        for (; s + 4 <= a.splits; s += 4) {
-          s0 += src[(int64_t)s * nw];
-          s1 += src[(int64_t)(s + 1) * nw];
-          s2 += src[(int64_t)(s + 2) * nw];
-          s3 += src[(int64_t)(s + 3) * nw];
-        }
-        for (; s < a.splits; ++s) s0 += src[(int64_t)s * nw];
-        const float gsum = (s0 + s1) + (s2 + s3);
+        const float gsum = strided_sum(a.part + poff + loc, nw, 0, 1, a.splits);
         a.grads[off + loc] = gsum;
         if constexpr (ADAM) {
-          const float wi = adam_apply(a.adam, adam_schedule(a.adam), off + loc, gsum);
+          const float wi = adam_apply(a.adam, sched, off + loc, gsum);
           pack_weight(a.adam.packed, W, a.depth, l, fan, woff, (int)(loc / fan), (int)(loc % fan), wi);
         }
         return;
@@ -549,23 +630,14 @@ __global__ void __launch_bounds__(256) train_reduce_kernel(ReduceArgs a) {
       dst = out_off + (int64_t)a.O * W + o;
     }
   }
-  float s0 = 0.0f, s1 = 0.0f;
-  if (src) {
-    int k = ph;
-    for (; k + 16 < a.tiles; k += 32) {  // two independent chains keep two loads in flight
-      s0 += src[(int64_t)k * stride];
-      s1 += src[(int64_t)(k + 16) * stride];
-    }
-    if (k < a.tiles) s0 += src[(int64_t)k * stride];
-  }
-  ps[ph][qi] = s0 + s1;
+  ps[ph][qi] = src ? strided_sum(src, stride, ph, 16, a.tiles) : 0.0f;
   __syncthreads();
   if (ph == 0 && src) {
     float t = 0.0f;
 #pragma unroll
     for (int j = 0; j < 16; ++j) t += ps[j][qi];
     a.grads[dst] = t;
-    if constexpr (ADAM) adam_apply(a.adam, adam_schedule(a.adam), dst, t);
+    if constexpr (ADAM) adam_apply(a.adam, sched, dst, t);
   }
 }
 
