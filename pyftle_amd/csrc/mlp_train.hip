@@ -487,14 +487,21 @@ __device__ __forceinline__ AdamSchedule adam_schedule(const AdamParams &p) {
   return s;
 }
 
-// updates parameter i with gradient g; returns the new weight
-__device__ __forceinline__ float adam_apply(const AdamParams &p, const AdamSchedule &s, int64_t i, float g) {
-  const float gi = g + p.wd * p.w[i];
-  const float mi = p.b1 * p.m[i] + (1.0f - p.b1) * gi;
-  const float vi = p.b2 * p.v[i] + (1.0f - p.b2) * gi * gi;
+// Parameter i's weight and moments, loaded ahead of the gradient they are updated with.
+struct AdamState {
+  float w, m, v;
+};
+__device__ __forceinline__ AdamState adam_load(const AdamParams &p, int64_t i) { return {p.w[i], p.m[i], p.v[i]}; }
+
+// updates parameter i (state st, as loaded) with gradient g; returns the new weight
+__device__ __forceinline__ float adam_apply(const AdamParams &p, const AdamSchedule &s, int64_t i, float g,
+                                            AdamState st) {
+  const float gi = g + p.wd * st.w;
+  const float mi = p.b1 * st.m + (1.0f - p.b1) * gi;
+  const float vi = p.b2 * st.v + (1.0f - p.b2) * gi * gi;
   p.m[i] = mi;
   p.v[i] = vi;
-  const float wi = p.w[i] - s.lr * (mi / s.bc1) / (sqrtf(vi / s.bc2) + p.eps);
+  const float wi = st.w - s.lr * (mi / s.bc1) / (sqrtf(vi / s.bc2) + p.eps);
   p.w[i] = wi;
   return wi;
 }
@@ -589,10 +596,12 @@ __global__ void __launch_bounds__(256) train_reduce_kernel(ReduceArgs a) {
       const int64_t nw = (int64_t)W * fan;
       if (q < woff + nw) {
         const int64_t loc = q - woff;
+        AdamState st{};  // loaded with the partials: one memory round trip, not two
+        if constexpr (ADAM) st = adam_load(a.adam, off + loc);
         const float gsum = strided_sum(a.part + poff + loc, nw, 0, 1, a.splits);
         a.grads[off + loc] = gsum;
         if constexpr (ADAM) {
-          const float wi = adam_apply(a.adam, sched, off + loc, gsum);
+          const float wi = adam_apply(a.adam, sched, off + loc, gsum, st);
           pack_weight(a.adam.packed, W, a.depth, l, fan, woff, (int)(loc / fan), (int)(loc % fan), wi);
         }
         return;
@@ -630,6 +639,10 @@ __global__ void __launch_bounds__(256) train_reduce_kernel(ReduceArgs a) {
       dst = out_off + (int64_t)a.O * W + o;
     }
   }
+  AdamState st{};
+  if constexpr (ADAM) {
+    if (ph == 0 && src) st = adam_load(a.adam, dst);
+  }
   ps[ph][qi] = src ? strided_sum(src, stride, ph, 16, a.tiles) : 0.0f;
   __syncthreads();
   if (ph == 0 && src) {
@@ -637,7 +650,7 @@ __global__ void __launch_bounds__(256) train_reduce_kernel(ReduceArgs a) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) t += ps[j][qi];
     a.grads[dst] = t;
-    if constexpr (ADAM) adam_apply(a.adam, sched, dst, t);
+    if constexpr (ADAM) adam_apply(a.adam, sched, dst, t, st);
   }
 }
 
@@ -663,7 +676,7 @@ __global__ void __launch_bounds__(256) train_adam_kernel(AdamParams ad, const fl
                                                          int depth) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const float wi = adam_apply(ad, adam_schedule(ad), p, g[p]);
+  const float wi = adam_apply(ad, adam_schedule(ad), p, g[p], adam_load(ad, p));
   int64_t off = 0, pk = 0;
   for (int l = 0; l < depth; ++l) {
     const int fan = l == 0 ? kFeat : W;
