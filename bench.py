@@ -429,7 +429,8 @@ def _abc_cpu_workload(torch, wl, seeds, delta=1e-3):
                 steps=int(round(wl["T"] / wl["h"])), host=dict(vel=vel, n=n, sp=L / n))
 
 
-def build_surrogate3d(torch, n=192, ns=256, t0s=(0.0, 2.5, 5.0, 7.5), members=4, epochs=100, n_traj=4_000_000):
+def build_surrogate3d(torch, n=192, ns=256, t0s=(0.0, 2.5, 5.0, 7.5), members=4, epochs=100, n_traj=4_000_000,
+                      depth=6):
     """Unsteady ABC on a periodic 192^3 grid: 64 snapshots per period 10 (dt = 10/64),
     stored from t = -1.25 to 16.25 so that training windows start in [-1.25, 11.25)
     and the 4 inference windows (T = 5, RK4 h = 0.01) lie inside."""
@@ -438,12 +439,12 @@ def build_surrogate3d(torch, n=192, ns=256, t0s=(0.0, 2.5, 5.0, 7.5), members=4,
     dts = 10.0 / 64.0
     field = _abc_unsteady_field_on_device(torch, n, -1.25, 113, dts)
     return dict(name="surrogate3d", field=field, ns=ns, t0s=list(t0s), members=members, epochs=epochs,
-                n_traj=n_traj, T=5.0, h=0.01, dim=3, SF=SF,
+                n_traj=n_traj, T=5.0, h=0.01, dim=3, SF=SF, depth=depth,
                 config={"workload": "surrogate3d", "flow": "ABC, A(t)=sqrt3+0.5 sin(pi t/5), B=sqrt2, C=1",
                         "grid": [n, n, n], "periodic": True, "snapshots_per_period": 64,
                         "stored_time": [-1.25, 16.25], "seeds": [ns, ns, ns], "neighbors": 6, "delta": 1e-3,
                         "t0_values": list(t0s), "T": 5.0, "rk4_h": 0.01,
-                        "mlp": "64 Fourier freqs (sigma 0.5) on (x,y,z,t0), 6x512 SiLU -> 3",
+                        "mlp": f"64 Fourier freqs (sigma 0.5) on (x,y,z,t0), {depth}x512 SiLU -> 3",
                         "members": members, "epochs": epochs, "trajectories": n_traj,
                         "accuracy_reference": "RK4 on a 1% held-out seed subset (rng seed 1)",
                         "dtype": "bf16 MFMA, fp32 accumulate"})
@@ -466,8 +467,8 @@ def surrogate3d_measure(torch, wl, args, full=True):
                                  n_bins=400)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t
-    spec = S.MLPSpec(d_in=4, d_out=3, n_freq=64, width=512, depth=6, sigma=0.5, scale=(0.5, 0.5, 0.5, 0.1),
-                     out_scale=2.0)
+    spec = S.MLPSpec(d_in=4, d_out=3, n_freq=64, width=512, depth=wl.get("depth", 6), sigma=0.5,
+                     scale=(0.5, 0.5, 0.5, 0.1), out_scale=2.0)
     t = time.perf_counter()
     ens = SF.fit_ensemble(X, D, spec, members=wl["members"], epochs=wl["epochs"], batch=16384, lr=2e-3, log=log)
     torch.cuda.synchronize()
@@ -610,7 +611,7 @@ def surrogate3d_measure(torch, wl, args, full=True):
         specified["without_probe_plan"] = spec_runs["without_probe_plan"]
     summary.update({
         "workload": "surrogate3d", "seeds": n_seeds, "window_t0": t0w, "members": wl["members"],
-        "network": "64 Fourier freqs on (x,y,z,t0), 6x512 SiLU -> 3",
+        "network": f"64 Fourier freqs on (x,y,z,t0), {spec.depth}x512 SiLU -> 3",
         "gate": "ftle: max over members |FTLE_member - FTLE_ensemble| / (FTLE range of the window's probe); "
                 "displacement: max member deviation of the displacement, domain units (the domain is 2 pi)",
         "specified": specified, "best_within_2pct": best, "measured_all_seeds_first_window": measured,
@@ -1401,8 +1402,8 @@ def main():
     ap.add_argument("--surrogate-epochs", type=int, default=0,
                     help="surrogate2d / surrogate3d: training epochs (0 = the workload's default)")
     ap.add_argument("--surrogate-depth", type=int, default=0,
-                    help="surrogate2d: hidden layers of width 256 (0 = the 5 of the specified network; "
-                         "other depths are an accuracy/speed exploration)")
+                    help="surrogate2d / surrogate3d: hidden layers of width 256 / 512 (0 = the 5 / 6 of the "
+                         "specified networks; other depths are an accuracy/speed exploration)")
     ap.add_argument("--train-batch", type=int, default=0, help="mlp_train: rows per step (0 = 16384)")
     ap.add_argument("--train-graph", type=int, default=0, help="mlp_train: replay each step as a HIP graph")
     ap.add_argument("--train-shape", default="2d", choices=["2d", "3d"], help="mlp_train: the 2D or the 3D network")
@@ -1467,7 +1468,7 @@ def main():
     kw = {}
     if args.workload in ("surrogate2d", "surrogate3d") and args.surrogate_epochs > 0:
         kw["epochs"] = args.surrogate_epochs
-    if args.workload == "surrogate2d" and args.surrogate_depth > 0:
+    if args.workload in ("surrogate2d", "surrogate3d") and args.surrogate_depth > 0:
         kw["depth"] = args.surrogate_depth
     if args.workload == "mlp_train":
         kw["shape"] = args.train_shape
