@@ -26,13 +26,14 @@ constexpr int kThreads = 64 * kWaves;
 constexpr int kMaxDepth = 8;
 constexpr int kMaxIn = 4;
 
-// Shape of one instantiation: hidden width W, RT row tiles of 16 per work-group.
-template <int W, int RT>
+// Shape of one instantiation: hidden width W, RT row tiles of 16 per work-group, WV waves
+// (each owning kTilesPerWave of the layer's 16-feature tiles for all the rows).
+template <int W, int RT, int WV = kWaves>
 struct Shape {
   static constexpr int kFragTiles = W / 16;             // 16-feature tiles of a layer
-  static constexpr int kTilesPerWave = kFragTiles / kWaves;
+  static constexpr int kTilesPerWave = kFragTiles / WV;
   static constexpr int kRows = 16 * RT;
-  static_assert(W % 64 == 0 && W >= kFeat, "the width splits over 4 waves");
+  static_assert(W % (16 * WV) == 0 && W >= kFeat, "the width splits over the waves");
 };
 
 struct FusedArgs {
@@ -98,13 +99,13 @@ __device__ __forceinline__ float sigmoid_f(float z) {
 // each load has PD k-steps of the wave's MFMAs (interleaved with the other waves on the SIMD)
 // to land.  Every layer's k-step count must be a multiple of PD, so the slot of a k-step is
 // the same whichever layer prefetched it.
-template <int W, int RT, int K, int PD>
+template <int W, int RT, int K, int PD, int WV = kWaves>
 __device__ __forceinline__ void layer_mma_ring(__amdgpu_buffer_rsrc_t w, int lbytes, int next_lbytes,
                                                const __bf16 *act,
-                                               bf16x8_t (&ring)[PD][Shape<W, RT>::kTilesPerWave],
-                                               f32x4_t (&acc)[Shape<W, RT>::kTilesPerWave][RT], int ft0,
+                                               bf16x8_t (&ring)[PD][Shape<W, RT, WV>::kTilesPerWave],
+                                               f32x4_t (&acc)[Shape<W, RT, WV>::kTilesPerWave][RT], int ft0,
                                                int lane_in) {
-  constexpr int kTilesPerWave = Shape<W, RT>::kTilesPerWave, kRowTiles = RT;
+  constexpr int kTilesPerWave = Shape<W, RT, WV>::kTilesPerWave, kRowTiles = RT;
   constexpr int KS = K / 32;
   static_assert(KS % PD == 0, "the ring depth divides the k-steps of every layer");
   const int lane = opaque_lane(lane_in);

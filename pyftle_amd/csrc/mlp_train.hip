@@ -68,13 +68,13 @@ __host__ __device__ __forceinline__ int tlayer_bytes(int l, int depth) {
   return 2 * (W * kFeat + (depth - 1) * W * W) + 2 * (l - 1) * W * W;
 }
 
-// LDS rows [0, 16 RT) x features [0, nfeat) -> dst[f * ld + col0 + row] (bf16)
-template <int W, int RT>
+// LDS rows [0, 16 RT) x features [0, nfeat) -> dst[f * ld + col0 + row] (bf16), NTH threads
+template <int W, int RT, int NTH>
 __device__ __forceinline__ void transpose_out(const __bf16 *act, int nfeat, __bf16 *dst, int64_t ld,
                                               int64_t col0, int tid) {
   constexpr int Q = 2 * RT;  // row octets
   const int tasks = (nfeat / 2) * Q;
-  for (int task = tid; task < tasks; task += kThreads) {
+  for (int task = tid; task < tasks; task += NTH) {
     const int q = task % Q, p = task / Q;
     const int rot = (q >> 1) & 7;
     const int f = 2 * p;
@@ -173,13 +173,13 @@ __device__ __forceinline__ void store_transposed(bf16x4_t h0, bf16x4_t h1, __bf1
 
 // Per-feature sum over this lane's rows (its RT row tiles), then over the 16 lanes that
 // share the lane's features (lane % 16 = row within a tile): lanes with lane % 16 == 0 write.
-template <int W, int RT>
-__device__ __forceinline__ void feature_sums(const f32x4_t (&acc)[Shape<W, RT>::kTilesPerWave][RT], float *dst,
+template <int W, int RT, int WV>
+__device__ __forceinline__ void feature_sums(const f32x4_t (&acc)[Shape<W, RT, WV>::kTilesPerWave][RT], float *dst,
                                              int ft0, int lane_in) {
   const int lane = opaque_lane(lane_in);
   const int r = lane & 15, g = lane >> 4;
 #pragma unroll
-  for (int i = 0; i < Shape<W, RT>::kTilesPerWave; ++i) {
+  for (int i = 0; i < Shape<W, RT, WV>::kTilesPerWave; ++i) {
     float s[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -194,13 +194,13 @@ __device__ __forceinline__ void feature_sums(const f32x4_t (&acc)[Shape<W, RT>::
 
 // acc (FP32) as bf16 activation rows of LDS (the layout the MFMA loop reads) and, transposed,
 // as dst[f * ld + col0 + row]
-template <int W, int RT>
-__device__ __forceinline__ void store_rows(const f32x4_t (&acc)[Shape<W, RT>::kTilesPerWave][RT], __bf16 *act,
+template <int W, int RT, int WV>
+__device__ __forceinline__ void store_rows(const f32x4_t (&acc)[Shape<W, RT, WV>::kTilesPerWave][RT], __bf16 *act,
                                            __bf16 *dst, int64_t ld, int64_t col0, int ft0, int lane_in) {
   const int lane = opaque_lane(lane_in);
   const int r = lane & 15, g = lane >> 4;
 #pragma unroll
-  for (int i = 0; i < Shape<W, RT>::kTilesPerWave; ++i) {
+  for (int i = 0; i < Shape<W, RT, WV>::kTilesPerWave; ++i) {
     const int f0 = (ft0 + i) * 16 + 4 * g;
     const int off = act_off<W>(r, f0 >> 3) + (f0 & 7);
     bf16x4_t h[RT];
@@ -215,16 +215,20 @@ __device__ __forceinline__ void store_rows(const f32x4_t (&acc)[Shape<W, RT>::kT
   }
 }
 
-// Two work-groups per CU at width 256 (one's epilogues under the other's MFMAs); at width
-// 512 the accumulators alone take half the registers of two, so one.
-template <int W, int RT, int O>
-__global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(TrainArgs a) {
-  using Sh = Shape<W, RT>;
-  constexpr int TPW = Sh::kTilesPerWave, kRows = Sh::kRows;
+// WV waves of 64 lanes per work-group.  At width 256 eight waves share 64 rows, two per SIMD
+// (each weight fragment feeds 4 MFMAs, against 2 with four waves on 32 rows and two
+// work-groups per CU); at width 512 four waves on 64 rows, one per SIMD: the accumulators
+// alone take half the registers of two.
+template <int W, int RT, int O, int WV>
+__global__ void __launch_bounds__(64 * WV, W == 512 ? 1 : 2) train_kernel(TrainArgs a) {
+  using Sh = Shape<W, RT, WV>;
+  constexpr int TPW = Sh::kTilesPerWave, kRows = Sh::kRows, NTH = 64 * WV;
   static_assert(RT % 2 == 0, "row tiles leave transposed in pairs");
+  static_assert(NTH >= kNFreq * kMaxIn && NTH >= kRows * 3, "one thread per staged frequency and output");
   __shared__ __attribute__((aligned(16))) __bf16 act[kRows * W];
   __shared__ __attribute__((aligned(16))) float freq_s[kNFreq * kMaxIn];
-  __shared__ float red[kWaves][kRows][4];
+  __shared__ float red[WV][kRows][4];
+  constexpr int kDPerWave = TPW * RT * 64 * 4;  // SiLU'(z) values (bf16) per wave and layer
   __shared__ float dy_s[kRows][4];
   __shared__ float sq_s[kRows * 4];
 
@@ -236,7 +240,7 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
   const int64_t col0 = (int64_t)tile * kRows;                 // its column in the step's buffers
   const int64_t M = a.M;
   if (a.step && tile == 0 && tid == 0) *a.step += 1;
-  {
+  if (tid < kNFreq * kMaxIn) {
     const int q = tid >> 2, c = tid & 3;
     float fv = 0.0f, sc = 0.0f;
     if (c < a.d_in) {
@@ -246,9 +250,9 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
     freq_s[tid] = fv * sc;
   }
   const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc((void *)a.Wp, 0, a.wp_bytes, 0x00020000);
-  // weight fragments PD k-steps ahead: at width 256 a k-step is only 2 RT MFMAs per
-  // fragment, so one k-step of cover leaves the L2 latency exposed; width 512 has no
-  // registers left for more
+  // weight fragments PD k-steps ahead: at width 256 a k-step is only TPW * RT = 8 MFMAs per
+  // wave, so one k-step of cover leaves the L2 latency exposed; width 512 has no registers
+  // left for more
   constexpr int PD = W == 256 ? 4 : 1;
   bf16x8_t ring[PD][TPW];
 #pragma unroll
@@ -258,7 +262,7 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
   __syncthreads();
 
   // Fourier features [sin | cos] of the tile's rows
-  for (int task = tid; task < kRows * 8; task += kThreads) {
+  for (int task = tid; task < kRows * 8; task += NTH) {
     const int j = task >> 3, fb = task & 7;
     float xin[kMaxIn];
 #pragma unroll
@@ -278,19 +282,19 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
     *reinterpret_cast<bf16x8_t *>(act + act_off<W>(j, kNFreq / 8 + fb)) = cs;
   }
   __syncthreads();
-  transpose_out<W, RT>(act, kFeat, a.FT, M, col0, tid);
+  transpose_out<W, RT, NTH>(act, kFeat, a.FT, M, col0, tid);
 
   f32x4_t acc[TPW][RT];
   // Forward.  Layer 0 (K = 128) and the last layer are peeled off the loop so that no
   // accumulator array is carried around a loop holding two instantiations of the MFMA loop.
-  layer_mma_ring<W, RT, kFeat, PD>(wr, layer_bytes<W>(0), layer_bytes<W>(1), act, ring, acc, ft0, lane);
+  layer_mma_ring<W, RT, kFeat, PD, WV>(wr, layer_bytes<W>(0), layer_bytes<W>(1), act, ring, acc, ft0, lane);
   for (int L = 0;; ++L) {
     __syncthreads();  // every wave has read this layer's inputs
     {
       const float *bl = a.params + b_off<W>(L);
       const int ln = opaque_lane(lane);
       const int r = ln & 15, g = ln >> 4;
-      __bf16 *dact = a.DACT + ((((int64_t)L * ntiles + tile) * kWaves + wave) * TPW * RT * 64) * 4;
+      __bf16 *dact = a.DACT + (((int64_t)L * ntiles + tile) * WV + wave) * kDPerWave;
       __bf16 *ht = a.HT + (int64_t)L * W * M;
 #pragma unroll
       for (int i = 0; i < TPW; ++i) {
@@ -318,11 +322,11 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
     }
     __syncthreads();
     if (L + 2 == a.depth) break;
-    layer_mma_ring<W, RT, W, PD>(wr, layer_bytes<W>(L + 1), layer_bytes<W>(L + 2), act, ring, acc, ft0, lane);
+    layer_mma_ring<W, RT, W, PD, WV>(wr, layer_bytes<W>(L + 1), layer_bytes<W>(L + 2), act, ring, acc, ft0, lane);
   }
   {
     const int L = a.depth - 1;
-    layer_mma_ring<W, RT, W, PD>(wr, layer_bytes<W>(L), tlayer_bytes<W>(L, a.depth), act, ring, acc, ft0, lane);
+    layer_mma_ring<W, RT, W, PD, WV>(wr, layer_bytes<W>(L), tlayer_bytes<W>(L, a.depth), act, ring, acc, ft0, lane);
     const float *bl = a.params + b_off<W>(L);
     __syncthreads();  // every wave has read the last layer's inputs
     // output layer (FP32): y = SiLU(z + b) Wout^T + bout, then the MSE gradient
@@ -374,7 +378,7 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
       const int n = tid / O, o = tid % O;
       float y = bout[o];
 #pragma unroll
-      for (int w = 0; w < kWaves; ++w) y += red[w][n][o];
+      for (int w = 0; w < WV; ++w) y += red[w][n][o];
       const float diff = y - a.t[(grow0 + n) * O + o];
       dy_s[n][o] = 2.0f * diff * a.loss_scale;
       sq_s[tid] = diff * diff * a.loss_scale;
@@ -440,16 +444,16 @@ __global__ void __launch_bounds__(kThreads, W == 512 ? 1 : 2) train_kernel(Train
   }
   // backward through the hidden layers: acc holds dZ of layer l
   for (int l = a.depth - 1;; --l) {
-    feature_sums<W, RT>(acc, a.bias_part + ((int64_t)tile * a.depth + l) * W, ft0, lane);
+    feature_sums<W, RT, WV>(acc, a.bias_part + ((int64_t)tile * a.depth + l) * W, ft0, lane);
     // every wave finished reading act before the last barrier
-    store_rows<W, RT>(acc, act, a.DZT + (int64_t)l * W * M, M, col0, ft0, lane);
+    store_rows<W, RT, WV>(acc, act, a.DZT + (int64_t)l * W * M, M, col0, ft0, lane);
     if (l == 0) break;
     __syncthreads();
     const int lb = tlayer_bytes<W>(l, a.depth);
     const int nlb = l > 1 ? tlayer_bytes<W>(l - 1, a.depth) : layer_bytes<W>(0);
-    layer_mma_ring<W, RT, W, PD>(wr, lb, nlb, act, ring, acc, ft0, lane);  // dH of layer l - 1
+    layer_mma_ring<W, RT, W, PD, WV>(wr, lb, nlb, act, ring, acc, ft0, lane);  // dH of layer l - 1
     __syncthreads();
-    const __bf16 *dact = a.DACT + (((((int64_t)(l - 1) * ntiles + tile) * kWaves + wave) * TPW * RT * 64) * 4);
+    const __bf16 *dact = a.DACT + (((int64_t)(l - 1) * ntiles + tile) * WV + wave) * kDPerWave;
     const int ln = opaque_lane(lane);
 #pragma unroll
     for (int i = 0; i < TPW; ++i)
@@ -692,9 +696,9 @@ __global__ void __launch_bounds__(256) train_adam_kernel(AdamParams ad, const fl
   }
 }
 
-template <int W, int RT, int O>
+template <int W, int RT, int O, int WV>
 int launch_train(const TrainArgs &a, hipStream_t st) {
-  train_kernel<W, RT, O><<<a.M / (16 * RT), kThreads, 0, st>>>(a);
+  train_kernel<W, RT, O, WV><<<a.M / (16 * RT), 64 * WV, 0, st>>>(a);
   return check_launch("mlp_train_grads");
 }
 
@@ -723,10 +727,8 @@ WorkLayout work_layout(int64_t M, int W, int depth, int O, int splits, int rows_
   return w;
 }
 
-// rows per work-group: 32 at width 256, so that a 16384-row step spreads over 512 work-groups
-// and two share a CU (0.135 -> 0.121 ms per step; 16 rows: 0.156); 64 at width 512, where 32
-// rows spill at two work-groups per CU and run no faster (0.366 against 0.362 ms)
-int tile_rows(int w) { return w == 256 ? 32 : 64; }
+// rows per work-group: 64 (eight waves at width 256, four at width 512)
+int tile_rows(int) { return 64; }
 
 // forward layers then transposed layers 1..depth-1, bf16
 int packed_train_bytes(int W, int depth) { return 2 * (W * kFeat + 2 * (depth - 1) * W * W); }
@@ -829,11 +831,11 @@ int train_step_launches(const pyftle_mlp_train_args *t, const AdamParams *adam, 
   a.step = t->step;
   int rc;
   if (W == 256) {
-    rc = O == 1 ? launch_train<256, 2, 1>(a, st) : O == 2 ? launch_train<256, 2, 2>(a, st)
-                                                            : launch_train<256, 2, 3>(a, st);
+    rc = O == 1 ? launch_train<256, 4, 1, 8>(a, st) : O == 2 ? launch_train<256, 4, 2, 8>(a, st)
+                                                               : launch_train<256, 4, 3, 8>(a, st);
   } else {
-    rc = O == 1 ? launch_train<512, 4, 1>(a, st) : O == 2 ? launch_train<512, 4, 2>(a, st)
-                                                            : launch_train<512, 4, 3>(a, st);
+    rc = O == 1 ? launch_train<512, 4, 1, 4>(a, st) : O == 2 ? launch_train<512, 4, 2, 4>(a, st)
+                                                               : launch_train<512, 4, 3, 4>(a, st);
   }
   if (rc) return rc;
   // weight gradients: dW_l (W x fan) = DZT_l (W x M) HT_{l-1}^T, both K-contiguous, split over K,
