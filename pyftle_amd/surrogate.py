@@ -430,6 +430,83 @@ class FusedTrainStep:
                _p(self.packed), stream_handle())
 
 
+class _Trainer:
+    """One model's training run (``train``): shuffled copies of the data, the device-side row
+    offset, step index and loss accumulator, and the step's kernels."""
+
+    def __init__(self, model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int, batch: int,
+                 lr: float, lr_min: float, seed: int, graph: bool, fused: bool, splits: int):
+        n = X.shape[0]
+        if n < batch:
+            raise ValueError(f"need at least one batch of {batch} rows, got {n}")
+        dev = X.device
+        self.model, self.X, self.n, self.batch = model, X, n, batch
+        self.lr, self.lr_min, self.graph = lr, lr_min, graph
+        self.steps_per_epoch = n // batch
+        self.total = epochs * self.steps_per_epoch
+        self.gen = torch.Generator(device=dev).manual_seed(seed)
+        self.Xs, self.Ts = torch.empty_like(X), torch.empty_like(T)
+        self.xb = torch.empty(batch, X.shape[1], dtype=X.dtype, device=dev)
+        self.tb = torch.empty(batch, T.shape[1], dtype=T.dtype, device=dev)
+        self.base = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.ar = torch.arange(batch, dtype=torch.int64, device=dev)
+        self.k_dev = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.acc = torch.zeros(1, dtype=torch.float32, device=dev)
+        self.step0 = model.step
+        self.use_fused = fused and fused_train_supported(model.spec, batch)
+        if self.use_fused:
+            # targets of the network output, shuffled with the inputs each epoch; the step index
+            # is advanced by the gradient call before Adam reads it
+            self.Tn = (T / model.spec.out_scale).contiguous()
+            self.k_dev.fill_(-1)
+            self.fstep = FusedTrainStep(model, self.Xs, self.Ts, batch, self.base, self.acc, splits=splits,
+                                        step=self.k_dev)
+        else:
+            self.Tn = T
+        self.g = None
+        self.done = 0
+
+    def _adam(self):
+        m = self.model
+        N.call("pyftle_mlp_adam_scheduled", _p(m.params), _p(m.m), _p(m.v), _p(m.grads), m.n_params, 0.9, 0.999,
+               1e-8, 0.0, float(self.lr), float(self.lr_min), self.total, self.step0, _p(self.k_dev),
+               _p(m.params_bf16), stream_handle())
+
+    def _body(self):
+        if self.use_fused:
+            self.fstep.step(self.lr, self.lr_min, self.total, self.step0)  # also advances base and the step index
+            return
+        idx = self.base + self.ar
+        torch.index_select(self.Xs, 0, idx, out=self.xb)
+        torch.index_select(self.Ts, 0, idx, out=self.tb)
+        self.model.loss_and_backward(self.xb, self.tb, loss_out=self.acc)
+        self._adam()
+        self.base.add_(self.batch)
+
+    def begin_epoch(self):
+        perm = torch.randperm(self.n, generator=self.gen, device=self.X.device)
+        torch.index_select(self.X, 0, perm, out=self.Xs)
+        torch.index_select(self.Tn, 0, perm, out=self.Ts)
+        self.base.zero_()
+        self.acc.zero_()
+
+    def step(self):
+        if self.graph and self.g is None and self.done >= 2:
+            self.g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.g):
+                self._body()
+        if self.g is not None:
+            self.g.replay()
+        else:
+            self._body()
+        self.done += 1
+
+    def finish(self):
+        self.model.step += self.total
+        if self.use_fused:
+            self.model.sync_bf16()  # the fused steps wrote the packed weights only
+
+
 def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20, batch: int = 32768,
           lr: float = 2e-3, lr_min: float = 2e-5, seed: int = 0, log=None, graph: bool = False,
           fused: bool = True, splits: int = 0):
@@ -438,82 +515,66 @@ def train(model: FlowMapMLP, X: torch.Tensor, T: torch.Tensor, epochs: int = 20,
     Each epoch gathers a shuffled copy of the data once; a step takes the next ``batch``
     rows at a device-side offset, runs forward, backward and the device-scheduled Adam
     update (``pyftle_mlp_adam_scheduled``), so no launch argument changes between steps.
-    ``fused`` (default, where ``fused_train_supported``): forward and backward in the fused
-    training kernels (``FusedTrainStep``: one kernel per row tile for the activations, the
-    split-K weight-gradient GEMMs, one reduction), about 10 launches a step; otherwise the
-    layer-by-layer kernels (~45 launches).
+    ``fused`` (default, where ``fused_train_supported``): forward, backward and Adam in the
+    fused training kernels (``FusedTrainStep``: one kernel per row tile for the activations,
+    the split-K weight-gradient GEMMs, one reduction that also applies Adam), three launches
+    a step; otherwise the layer-by-layer kernels (~45 launches).
     With ``graph`` the step is captured in a HIP graph after two eager steps and replayed;
     eager and replayed steps run the same kernels on the same buffers (bit-identical
     weights).  ``splits``: split-K of the fused step's weight-gradient GEMMs (0: its default).
     """
-    n = X.shape[0]
-    if n < batch:
-        raise ValueError(f"need at least one batch of {batch} rows, got {n}")
-    dev = X.device
-    steps_per_epoch = n // batch
-    total = epochs * steps_per_epoch
-    gen = torch.Generator(device=dev).manual_seed(seed)
-    Xs, Ts = torch.empty_like(X), torch.empty_like(T)
-    xb = torch.empty(batch, X.shape[1], dtype=X.dtype, device=dev)
-    tb = torch.empty(batch, T.shape[1], dtype=T.dtype, device=dev)
-    base = torch.zeros(1, dtype=torch.int64, device=dev)
-    ar = torch.arange(batch, dtype=torch.int64, device=dev)
-    k_dev = torch.zeros(1, dtype=torch.int32, device=dev)
-    acc = torch.zeros(1, dtype=torch.float32, device=dev)
-    step0 = model.step
-
-    use_fused = fused and fused_train_supported(model.spec, batch)
-    if use_fused:
-        # targets of the network output, shuffled with the inputs each epoch; the step index
-        # is advanced by the gradient call before Adam reads it
-        Tn = (T / model.spec.out_scale).contiguous()
-        k_dev.fill_(-1)
-        fstep = FusedTrainStep(model, Xs, Ts, batch, base, acc, splits=splits, step=k_dev)
-    else:
-        Tn = T
-
-    def adam():
-        N.call("pyftle_mlp_adam_scheduled", _p(model.params), _p(model.m), _p(model.v), _p(model.grads),
-               model.n_params, 0.9, 0.999, 1e-8, 0.0, float(lr), float(lr_min), total, step0, _p(k_dev),
-               _p(model.params_bf16), stream_handle())
-
-    def body():
-        if use_fused:
-            fstep.step(lr, lr_min, total, step0)  # also advances base and the step index
-            return
-        idx = base + ar
-        torch.index_select(Xs, 0, idx, out=xb)
-        torch.index_select(Ts, 0, idx, out=tb)
-        model.loss_and_backward(xb, tb, loss_out=acc)
-        adam()
-        base.add_(batch)
-
-    g = None
+    tr = _Trainer(model, X, T, epochs, batch, lr, lr_min, seed, graph, fused, splits)
     losses = []
-    done = 0
     for ep in range(epochs):
-        perm = torch.randperm(n, generator=gen, device=dev)
-        torch.index_select(X, 0, perm, out=Xs)
-        torch.index_select(Tn, 0, perm, out=Ts)
-        base.zero_()
-        acc.zero_()
-        for i in range(steps_per_epoch):
-            if graph and g is None and done >= 2:
-                g = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(g):
-                    body()
-            if g is not None:
-                g.replay()
-            else:
-                body()
-            done += 1
-        losses.append(acc.item() / steps_per_epoch)
+        tr.begin_epoch()
+        for _ in range(tr.steps_per_epoch):
+            tr.step()
+        losses.append(tr.acc.item() / tr.steps_per_epoch)
         if log:
             log(f"epoch {ep} loss {losses[-1]:.3e}")
-    model.step += total
-    if use_fused:
-        model.sync_bf16()  # the fused steps wrote the packed weights only
+    tr.finish()
     return losses
+
+
+def train_ensemble(models: list[FlowMapMLP], X: torch.Tensor, T: torch.Tensor, epochs: int = 20,
+                   batch: int = 32768, lr: float = 2e-3, lr_min: float = 2e-5, seeds=None, fused: bool = True,
+                   splits: int = 0) -> list[list[float]]:
+    """``train`` of several models on the same data at once: model k's run is its own
+    ``train(models[k], X, T, ..., seed=seeds[k])`` (the same weights bit for bit), with the
+    runs' steps interleaved, each run on a stream of its own (default priority), so that one
+    run's kernels fill what another's leave idle.  Per-epoch losses stay on the device until
+    the end (no synchronization between epochs).  Returns each model's per-epoch losses."""
+    seeds = list(range(len(models))) if seeds is None else list(seeds)
+    if len(seeds) != len(models):
+        raise ValueError("one seed per model")
+    dev = X.device
+    cur = torch.cuda.current_stream(dev)
+    streams = [torch.cuda.Stream(device=dev) for _ in models]
+    trainers, hist = [], []
+    for m, st, sd in zip(models, streams, seeds):
+        st.wait_stream(cur)
+        with torch.cuda.stream(st):
+            trainers.append(_Trainer(m, X, T, epochs, batch, lr, lr_min, sd, False, fused, splits))
+            hist.append(torch.zeros(epochs, dtype=torch.float32, device=dev))
+    for ep in range(epochs):
+        for tr, st in zip(trainers, streams):
+            with torch.cuda.stream(st):
+                tr.begin_epoch()
+        for _ in range(trainers[0].steps_per_epoch):
+            for tr, st in zip(trainers, streams):
+                with torch.cuda.stream(st):
+                    tr.step()
+        for tr, st, h in zip(trainers, streams, hist):
+            with torch.cuda.stream(st):
+                h[ep:ep + 1].copy_(tr.acc)
+    out = []
+    for tr, st, h in zip(trainers, streams, hist):
+        with torch.cuda.stream(st):
+            tr.finish()
+            vals = h.cpu().tolist()
+        cur.wait_stream(st)
+        out.append([v / tr.steps_per_epoch for v in vals])
+    return out
 
 
 class FlowMapEnsemble:

@@ -42,7 +42,7 @@ import torch
 from . import window as W
 from .batch import BatchConfig, BatchPlan, Window
 from .field import DeviceField
-from .surrogate import FlowMapEnsemble, FlowMapMLP, MLPSpec, ftle_from_displacement_jacobian, train
+from .surrogate import FlowMapEnsemble, FlowMapMLP, MLPSpec, ftle_from_displacement_jacobian, train, train_ensemble
 
 
 def uniform_seeds(n: int, lower, upper, gen: torch.Generator, device="cuda") -> torch.Tensor:
@@ -77,16 +77,23 @@ def make_training_data(field: DeviceField, n_traj: int, T: float, h: float, t0_r
 
 
 def fit_ensemble(X, D, spec: MLPSpec, members: int = 4, epochs: int = 20, batch: int = 32768,
-                 lr: float = 2e-3, log=None) -> FlowMapEnsemble:
-    models = []
-    for k in range(members):
-        m = FlowMapMLP(spec, seed=1000 + k)
-        t0 = time.perf_counter()
-        losses = train(m, X, D, epochs=epochs, batch=batch, lr=lr, seed=k)
-        torch.cuda.synchronize()
-        if log:
-            log(f"member {k}: loss {losses[0]:.3e} -> {losses[-1]:.3e} in {time.perf_counter() - t0:.1f} s")
-        models.append(m)
+                 lr: float = 2e-3, log=None, concurrent: bool = True) -> FlowMapEnsemble:
+    """Members ``FlowMapMLP(spec, seed=1000 + k)`` trained on (X, D), member k shuffling with
+    seed k.  ``concurrent`` (default): the members' runs interleaved on a stream each
+    (``train_ensemble``); otherwise one after another (``train``).  The weights are the same
+    bit for bit either way."""
+    models = [FlowMapMLP(spec, seed=1000 + k) for k in range(members)]
+    t0 = time.perf_counter()
+    if concurrent:
+        all_losses = train_ensemble(models, X, D, epochs=epochs, batch=batch, lr=lr, seeds=range(members))
+    else:
+        all_losses = [train(m, X, D, epochs=epochs, batch=batch, lr=lr, seed=k) for k, m in enumerate(models)]
+    torch.cuda.synchronize()
+    if log:
+        for k, losses in enumerate(all_losses):
+            log(f"member {k}: loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+        log(f"{members} members trained in {time.perf_counter() - t0:.1f} s"
+            f"{' (concurrently)' if concurrent else ''}")
     return FlowMapEnsemble(models)
 
 

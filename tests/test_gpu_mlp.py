@@ -297,6 +297,33 @@ def test_fused_rejects_unsupported_shapes():
         m.forward_fused(torch.zeros(4, 3, device="cuda"))
 
 
+@pytest.mark.parametrize("fused", [True, False])
+def test_concurrent_ensemble_training_equals_sequential(fused):
+    """train_ensemble (the members' runs interleaved, a stream each) leaves every member's
+    parameters, Adam moments and bf16 weights, and (fused) its per-epoch losses, exactly as its
+    own train() call does."""
+    from pyftle_amd.surrogate import train, train_ensemble
+
+    g = torch.Generator(device="cuda").manual_seed(8)
+    X = torch.rand(8192, 3, device="cuda", generator=g) * torch.tensor([2.0, 1.0, 10.0], device="cuda")
+    T = torch.stack([torch.sin(np.pi * X[:, 0]) * 0.3, torch.cos(np.pi * X[:, 1]) * 0.2], 1)
+    seq = [_train_model(256, 3, 3, 2, seed=20 + k) for k in range(3)]
+    con = [_train_model(256, 3, 3, 2, seed=20 + k) for k in range(3)]
+    l_seq = [train(m, X, T, epochs=3, batch=1024, lr=3e-3, seed=k, fused=fused) for k, m in enumerate(seq)]
+    l_con = train_ensemble(con, X, T, epochs=3, batch=1024, lr=3e-3, seeds=range(3), fused=fused)
+    torch.cuda.synchronize()
+    for k in range(3):
+        assert torch.equal(seq[k].params, con[k].params), k
+        assert torch.equal(seq[k].m, con[k].m) and torch.equal(seq[k].v, con[k].v), k
+        assert torch.equal(seq[k].params_bf16, con[k].params_bf16), k
+        if fused:
+            assert l_seq[k] == l_con[k], k
+        else:  # the layered path adds its loss partials with atomics: the last bit follows their order
+            np.testing.assert_allclose(l_con[k], l_seq[k], rtol=1e-6)
+        assert seq[k].step == con[k].step == 24
+    assert not torch.equal(con[0].params, con[1].params)
+
+
 def test_graph_captured_training_equals_eager():
     """The training step replayed from a HIP graph (with eager shuffles and host syncs
     between replays) updates the weights exactly as the same step run eagerly."""
