@@ -207,16 +207,25 @@ class SurrogateFTLE:
         n, dim = seeds.shape
         m = max(1, n // self.probe)
         per_axis = max(1, min(self.tiles, int((m / 32.0) ** (1.0 / dim))))
-        fin = torch.isfinite(seeds)  # non-finite coordinates fall in tile 0 of their axis
-        lo = torch.where(fin, seeds, float("inf")).min(0).values
-        hi = torch.where(fin, seeds, float("-inf")).max(0).values
-        lo = torch.where(torch.isfinite(lo), lo, torch.zeros_like(lo))
+        # the seeds' box in one pass; a non-finite coordinate shows in it (NaN or +-inf), and
+        # only then the box is taken over the finite ones (non-finite coordinates fall in tile 0
+        # of their axis)
+        lo, hi = W.column_box(seeds)
+        finite = bool(torch.isfinite(lo).all() and torch.isfinite(hi).all())
+        if not finite:
+            fin = torch.isfinite(seeds)
+            lo = torch.where(fin, seeds, float("inf")).min(0).values
+            hi = torch.where(fin, seeds, float("-inf")).max(0).values
+            lo = torch.where(torch.isfinite(lo), lo, torch.zeros_like(lo))
         ext = (hi - lo).nan_to_num(1.0, 1.0, 1.0).clamp_min(1e-300)
-        pos = ((seeds - lo) / ext * per_axis).nan_to_num(0.0, 0.0, 0.0)
-        cell = pos.floor().clamp_(0, per_axis - 1).long()
-        tid = torch.zeros(n, dtype=torch.long, device=seeds.device)
-        for a in range(dim):
-            tid = tid * per_axis + cell[:, a]
+        cell = torch.sub(seeds, lo).div_(ext).mul_(per_axis)  # one temporary, updated in place
+        if not finite:
+            cell.nan_to_num_(0.0, 0.0, 0.0)
+        cell.floor_().clamp_(0, per_axis - 1)
+        strides = torch.tensor([float(per_axis ** (dim - 1 - a)) for a in range(dim)], dtype=cell.dtype,
+                               device=seeds.device)
+        tid = cell.mul_(strides).sum(1).long()  # exact: small integers
+        del cell
         k = torch.arange(m, dtype=torch.long, device=seeds.device)
         idx = (k * _PROBE_MULT) % n if math.gcd(_PROBE_MULT, n) == 1 else k * self.probe
         n_tiles = per_axis ** dim
