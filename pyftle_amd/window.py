@@ -41,6 +41,8 @@ def column_box(x: torch.Tensor):
     tall, narrow tensor is slow along dim 0 (16.8M x 3: 9.7 ms), so the rows are reduced 64 at a
     time as one row of 64 d values (0.17 ms), the 64 partial results folded afterwards."""
     n, d = x.shape
+    if n == 0:
+        return torch.aminmax(x, dim=0)  # torch's own error for an empty reduction
     b = 64
     nb = n // b
     parts = []
