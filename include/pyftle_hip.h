@@ -380,7 +380,7 @@ int pyftle_batch_run_noalloc(const pyftle_batch_field *fields, int32_t n_fields,
 /* ---- flow-map surrogate: Fourier-feature MLP kernels ----
  * No reference counterpart: a learned approximation of the flow map that
  * pyFTLE's RK4 path (integrate.py:265-342) computes; RK4 stays the fallback
- * and the judge.  BF16 operands, FP32 accumulation (MFMA 32x32x16). */
+ * and the accuracy reference.  BF16 operands, FP32 accumulation (MFMA 32x32x16). */
 #define PYFTLE_EPI_F32 0       /* C (FP32, split-K partials at C + split*M*ldc) = A B */
 #define PYFTLE_EPI_BIAS_SILU 1 /* Z = A B + bias (bf16, optional); C = SiLU(Z) (bf16) */
 #define PYFTLE_EPI_DSILU 2     /* C = (A B) * SiLU'(Z) (bf16), Z read at the same row modulo z_rows */
