@@ -122,7 +122,7 @@ class SurrogateFTLE:
 
     def __init__(self, ensemble: FlowMapEnsemble, field: DeviceField, T: float, h: float,
                  delta: float = 1e-3, tol: float = 1e-3, time_mode: str = "linear", chunk: int = 1 << 21,
-                 jacobian_members: int | None = None, gate: str = "ftle", probe: int = 64, tiles: int = 16,
+                 jacobian_members: int | None = None, gate: str = "ftle", probe: int = 256, tiles: int = 16,
                  tile_fallback_max: float = 0.5, plan_min_seeds: int = 1 << 16):
         if gate not in ("ftle", "displacement"):
             raise ValueError(f"unknown gate {gate!r}: 'ftle' or 'displacement'")
@@ -323,15 +323,18 @@ class SurrogateFTLE:
             return results
         bads = torch.stack([r.fallback for r in results])        # (starts, seeds)
         counts = bads.sum(1).tolist()                            # host sync 1
-        flat = torch.nonzero(bads.view(-1)).flatten()            # host sync 2
+        # the fallback seeds' indices, needed only where a start falls back in part
+        flat = torch.nonzero(bads.view(-1)).flatten() if any(0 < c < n for c in counts) else None  # sync 2
         wins, where = [], []
         all_pos = all_order = None  # the whole seed set's neighbour groups, built once when needed
         off = 0
         for k, (t0, c) in enumerate(zip(t0s, counts)):
             if not c:
                 continue
-            idx = flat[off:off + c] - k * n
-            off += c
+            idx = None  # the whole start
+            if flat is not None:
+                idx = flat[off:off + c] - k * n
+                off += c
             if c == n:
                 if all_pos is None:
                     all_pos = neighbour_groups(seeds, self.delta)
@@ -347,7 +350,10 @@ class SurrogateFTLE:
             batch = BatchPlan(wins, method="rk4", time_mode=self.time_mode, cfg=cfg)
             batch.run()
             for w, (k, idx) in zip(wins, where):
-                results[k].ftle[idx] = w.ftle
+                if idx is None:
+                    results[k].ftle.copy_(w.ftle)
+                else:
+                    results[k].ftle[idx] = w.ftle
             batch.close()  # waits for the device
         return results
 

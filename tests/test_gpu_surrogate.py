@@ -243,7 +243,7 @@ def test_ftle_gate_units_and_forced_fallback(gyre_setup):
 
 
 def test_probe_plan_keeps_the_per_seed_gate(gyre_setup):
-    """The probe plan (65,536 seeds, probe every 64th, 5x5 tiles): every seed returned from
+    """The probe plan (65,536 seeds, probe every 64th here, 5x5 tiles): every seed returned from
     the ensemble passed its own gate and carries the ensemble's value for that seed; every
     other seed is RK4 bit for bit; tiles sent to RK4 whole are those whose probed seeds
     failed beyond the share; a tolerance nothing passes sends the whole field to RK4 and one
@@ -256,7 +256,7 @@ def test_probe_plan_keeps_the_per_seed_gate(gyre_setup):
     f_all, dis_all = plain.predict(seeds, 0.5)
     ref = plain.rk4_ftle(seeds, 0.5)
     tol = float(dis_all.median())
-    sf = SF.SurrogateFTLE(ens, f, T, h, gate="ftle", tol=tol, tile_fallback_max=0.5)
+    sf = SF.SurrogateFTLE(ens, f, T, h, gate="ftle", tol=tol, tile_fallback_max=0.5, probe=64)
     assert sf._planned(seeds)
     plan = sf._tiles_of(seeds)
     tid, pidx, n_tiles = plan.tile, plan.probe, plan.n_tiles

@@ -481,7 +481,11 @@ def test_surrogate_probe_tiles_cover_the_seeds():
     ys = (torch.arange(1024, dtype=torch.float64) + 0.5) / 1024.0
     cy, cx = torch.meshgrid(ys, xs, indexing="ij")
     seeds = torch.stack([cx.reshape(-1), cy.reshape(-1)], 1)
-    sf = SurrogateFTLE(None, None, 10.0, 0.05)
+    # the default spacing (every 256th seed): 8192 probes, 16 x 16 tiles of 32
+    plan = SurrogateFTLE(None, None, 10.0, 0.05)._tiles_of(seeds)
+    assert plan.n_tiles == 256 and plan.probe.numel() == seeds.shape[0] // 256
+    assert int(torch.bincount(plan.tile[plan.probe], minlength=256).min()) >= 20
+    sf = SurrogateFTLE(None, None, 10.0, 0.05, probe=64)
     plan = sf._tiles_of(seeds)
     tid, idx, n_tiles = plan.tile, plan.probe, plan.n_tiles
     assert n_tiles == 256 and int(plan.tile_seeds.sum()) == seeds.shape[0] and int(tid.min()) == 0 and int(tid.max()) == 255
