@@ -298,10 +298,11 @@ class SurrogateFTLE:
 
     def ftle(self, seeds: torch.Tensor, t0: float) -> SurrogateResult:
         f, bad, spread = self._window(seeds, t0, self._tiles_of(seeds) if self._planned(seeds) else None)
-        idx = torch.nonzero(bad).flatten()
-        if idx.numel() == seeds.shape[0]:  # the whole field from RK4, no gather
+        c = int(bad.sum())
+        if c == seeds.shape[0]:  # the whole field from RK4, no gather
             f = self.rk4_ftle(seeds, t0)
-        elif idx.numel():
+        elif c:
+            idx = torch.nonzero(bad).flatten()
             f[idx] = self.rk4_ftle(seeds[idx].contiguous(), t0)
         return SurrogateResult(ftle=f, fallback=bad, spread=spread)
 
