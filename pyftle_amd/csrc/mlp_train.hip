@@ -5,7 +5,9 @@
 // (integrate.py:265-342) is this engine's addition.
 //
 // 1. train_kernel (one work-group per tile of 16*RT rows, activations in LDS as in the
-//    fused inference kernel): Fourier features, every hidden layer forward, the output
+//    fused inference kernel; 64 rows, on eight waves at width 256 and four at width 512,
+//    each wave owning a share of the layer's 16-feature tiles for all the rows, so a weight
+//    fragment feeds RT MFMAs): Fourier features, every hidden layer forward, the output
 //    layer and the MSE gradient, then the backward pass through the hidden layers with the
 //    transposed weights (dH_{l-1} = dZ_l W_l, the same MFMA loop with W_l^T packed), each
 //    dZ_l = dH_l * SiLU'(z_l).  It writes what the weight gradients need, transposed so that
