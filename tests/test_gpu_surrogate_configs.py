@@ -11,8 +11,10 @@
 Checked: the ensemble's FTLE (mean Jacobian) within 2% RMSE of the field range against RK4;
 a forced fallback (negative tolerance) equal to RK4 bit for bit; the specified gate (FTLE
 disagreement 1e-3 of the range) giving a composite field within the bar; the fastest gated
-setting the bench reports (displacement gate with one Jacobian member: 0.03 in 2D, 1.0 in 3D,
-domain units) within the bar, its fallback seeds equal to RK4.
+setting on all four members (displacement gate with one Jacobian member: 0.03 in 2D, 1.0 in 3D,
+domain units) within the bar, its fallback seeds equal to RK4; the gate on the first two members
+(0.03 and 0.1 in 2D, 0.3 and 1.0 in 3D: the bench's fastest settings within the bar) within the
+bar at every window start, its fallback seeds equal to RK4.
 """
 
 import math
@@ -52,6 +54,10 @@ def test_surrogate2d_config_four_members():
     seeds = torch.stack([cx.reshape(-1), cy.reshape(-1)], 1).contiguous()
     sf = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], gate="ftle", tol=1e-3)
     sf_d = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], gate="displacement", tol=0.03, jacobian_members=1)
+    # the gate on the ensemble's first two members (the bench's fastest settings within the bar)
+    two = S.FlowMapEnsemble(ens.members[:2])
+    sf_2 = {tol: SF.SurrogateFTLE(two, field, wl["T"], wl["h"], gate="displacement", tol=tol, jacobian_members=1)
+            for tol in (0.03, 0.1)}
     for t0 in (0.0, 2.5, 5.0, 7.5):
         ref = sf.rk4_ftle(seeds, t0)
         f, dis = sf.predict(seeds, t0)
@@ -69,6 +75,12 @@ def test_surrogate2d_config_four_members():
         print(f"[surrogate2d] t0 {t0}: displacement gate 0.03: fallback {res_d.fallback.float().mean().item():.4f}, "
               f"rmse/range {err_d:.4f}", flush=True)
         assert err_d <= 0.02 and torch.equal(res_d.ftle[res_d.fallback], ref[res_d.fallback])
+        for tol, s2 in sf_2.items():
+            res_2 = s2.ftle(seeds, t0)
+            err_2 = SF.relative_rmse(res_2.ftle, ref)
+            print(f"[surrogate2d] t0 {t0}: 2-member displacement gate {tol:g}: fallback "
+                  f"{res_2.fallback.float().mean().item():.4f}, rmse/range {err_2:.4f}", flush=True)
+            assert err_2 <= 0.02 and torch.equal(res_2.ftle[res_2.fallback], ref[res_2.fallback])
     sf.tol = -1.0
     forced = sf.ftle(seeds, 2.5)
     assert bool(forced.fallback.all()) and torch.equal(forced.ftle, sf.rk4_ftle(seeds, 2.5))
@@ -100,6 +112,10 @@ def test_surrogate3d_config_four_members():
     sf = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], gate="ftle", tol=1e-3, chunk=1 << 19)
     sf_d = SF.SurrogateFTLE(ens, field, wl["T"], wl["h"], gate="displacement", tol=1.0, jacobian_members=1,
                             chunk=1 << 19)
+    # the gate on the ensemble's first two members (the bench's fastest settings within the bar)
+    two = S.FlowMapEnsemble(ens.members[:2])
+    sf_2 = {tol: SF.SurrogateFTLE(two, field, wl["T"], wl["h"], gate="displacement", tol=tol, jacobian_members=1,
+                                  chunk=1 << 19) for tol in (0.3, 1.0)}
     for t0 in wl["t0s"]:
         ref = sf.rk4_ftle(held, t0)
         f, dis = sf.predict(seeds, t0)  # the whole seed grid (the gate's range is the window's)
@@ -118,6 +134,12 @@ def test_surrogate3d_config_four_members():
         print(f"[surrogate3d] t0 {t0}: displacement gate 1.0: fallback {res_d.fallback.float().mean().item():.4f}, "
               f"rmse/range {err_d:.4f}", flush=True)
         assert err_d <= 0.02 and torch.equal(res_d.ftle[res_d.fallback], ref[res_d.fallback])
+        for tol, s2 in sf_2.items():
+            res_2 = s2.ftle(held, t0)
+            err_2 = SF.relative_rmse(res_2.ftle, ref)
+            print(f"[surrogate3d] t0 {t0}: 2-member displacement gate {tol:g}: fallback "
+                  f"{res_2.fallback.float().mean().item():.4f}, rmse/range {err_2:.4f}", flush=True)
+            assert err_2 <= 0.02 and torch.equal(res_2.ftle[res_2.fallback], ref[res_2.fallback])
     sf.tol = -1.0
     forced = sf.ftle(held, wl["t0s"][1])
     assert bool(forced.fallback.all()) and torch.equal(forced.ftle, sf.rk4_ftle(held, wl["t0s"][1]))
